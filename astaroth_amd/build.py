@@ -1,0 +1,75 @@
+"""Build the HIP libraries of astaroth_amd in-tree (hipcc, gfx950).
+
+Produces astaroth_amd/lib/libastaroth_fp32.so and libastaroth_fp64.so from the same sources,
+with AcReal = float / double (the reference's DOUBLE_PRECISION build option). Every HIP source
+is compiled with packed-FP32 instructions disabled.
+"""
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIBDIR = os.path.join(HERE, "lib")
+OBJDIR = os.path.join(HERE, "_build")
+
+SOURCES = ["device.hip", "solve_direct.hip", "solve_tiled.hip", "aux_kernels.hip", "host.cc",
+           "adjoint.hip"]
+ARCH = os.environ.get("ASTAROTH_OFFLOAD_ARCH", "gfx950")
+
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument", "-Wno-unused-result",
+]
+
+
+def _hipcc():
+    for cand in (os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "bin", "hipcc"), "hipcc"):
+        if os.path.sep not in cand or os.path.exists(cand):
+            return cand
+    return "hipcc"
+
+
+def _compile(args):
+    src, obj, defs = args
+    cmd = [_hipcc()] + HIP_FLAGS + defs + ["-I", os.path.join(REPO, "include"), "-c"]
+    if src.endswith(".cc"):
+        cmd += ["-x", "hip"]
+    cmd += [src, "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {src}:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return obj
+
+
+def build(verbose=True, jobs=None):
+    os.makedirs(LIBDIR, exist_ok=True)
+    os.makedirs(OBJDIR, exist_ok=True)
+    sources = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    tasks = []
+    for prec, dp in (("fp32", "0"), ("fp64", "1")):
+        for s in sources:
+            obj = os.path.join(OBJDIR, f"{os.path.splitext(s)[0]}_{prec}.o")
+            tasks.append((os.path.join(CSRC, s), obj, [f"-DAC_DOUBLE_PRECISION={dp}"]))
+    jobs = jobs or min(8, os.cpu_count() or 4)
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(_compile, tasks))
+    for prec in ("fp32", "fp64"):
+        pobjs = [o for o in objs if o.endswith(f"_{prec}.o")]
+        out = os.path.join(LIBDIR, f"libastaroth_{prec}.so")
+        tmp = out + ".tmp"
+        cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC"] + pobjs + ["-o", tmp]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        os.replace(tmp, out)
+        if verbose:
+            print(f"built {out}")
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

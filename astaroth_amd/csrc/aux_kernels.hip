@@ -1,0 +1,433 @@
+// aux_kernels.hip -- periodic halo fill (one launch for all fields), FP64-accumulating
+// reductions, one-pass diagnostics and small utility kernels.
+//
+// Periodic fill: ref acc-runtime/built-in/kernels.h:17-41 (BOUNDCOND_PERIODIC_DEVICE) launched
+// per field over the whole m^3 volume by ref src/core/device.cc:1009-1017. Here one launch
+// enumerates only the m^3 - n^3 halo cells of every field; the source is always an interior
+// cell, so the fill has no read/write hazards.
+// Reductions: ref src/core/kernels/reductions.cu:98-219 (map kernel into an n^3 scratchpad, then
+// hipcub DeviceReduce). Here the map is fused into a single pass with FP64 accumulation and a
+// deterministic two-stage tree (fixed grid, fixed order).
+#include "kernels.h"
+
+namespace acamd {
+
+constexpr int RED_TPB = 256;
+constexpr int RED_MAX_BLOCKS = 1024;
+
+// ---------------------------------------------------------------- periodic fill
+template <typename T>
+struct FillArgs {
+    T* f[NF];
+    int nx, ny, nz, mx, my;
+    long long halo_count;  // per field
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) periodic_fill_kernel(FillArgs<T> a)
+{
+    const long long t0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t0 >= a.halo_count)
+        return;
+    T* __restrict__ f = a.f[blockIdx.y];
+    const long long plane = (long long)a.mx * a.my;
+    int i, j, k;
+    long long t = t0;
+    if (t < 3 * plane) {
+        k = (int)(t / plane);
+        const long long r = t - (long long)k * plane;
+        j = (int)(r / a.mx);
+        i = (int)(r - (long long)j * a.mx);
+    }
+    else if (t < 6 * plane) {
+        t -= 3 * plane;
+        const int kk = (int)(t / plane);
+        const long long r = t - (long long)kk * plane;
+        k = a.nz + R + kk;
+        j = (int)(r / a.mx);
+        i = (int)(r - (long long)j * a.mx);
+    }
+    else {
+        t -= 6 * plane;
+        const long long per = 6LL * a.mx + 6LL * a.ny;
+        const int kk = (int)(t / per);
+        int r = (int)(t - (long long)kk * per);
+        k = R + kk;
+        if (r < 3 * a.mx) {
+            j = r / a.mx;
+            i = r - j * a.mx;
+        }
+        else if (r < 6 * a.mx) {
+            r -= 3 * a.mx;
+            const int jj = r / a.mx;
+            j = a.ny + R + jj;
+            i = r - jj * a.mx;
+        }
+        else {
+            r -= 6 * a.mx;
+            const int row = r / 6;
+            const int cc = r - row * 6;
+            j = R + row;
+            i = cc < 3 ? cc : a.nx + cc;
+        }
+    }
+    // proper modulo: also correct when an extent is smaller than the halo width
+    const int is = ((i - R) % a.nx + a.nx) % a.nx + R;
+    const int js = ((j - R) % a.ny + a.ny) % a.ny + R;
+    const int ks = ((k - R) % a.nz + a.nz) % a.nz + R;
+    const long long dst = (long long)i + (long long)a.mx * j + plane * k;
+    const long long src = (long long)is + (long long)a.mx * js + plane * ks;
+    f[dst] = f[src];
+}
+
+template <typename T>
+void launch_periodic_fill(T* const* fields, int nfields, int nx, int ny, int nz, hipStream_t s)
+{
+    FillArgs<T> a;
+    for (int q = 0; q < NF; ++q)
+        a.f[q] = q < nfields ? fields[q] : nullptr;
+    a.nx = nx;
+    a.ny = ny;
+    a.nz = nz;
+    a.mx = nx + 2 * R;
+    a.my = ny + 2 * R;
+    a.halo_count = 6LL * a.mx * a.my + (long long)nz * (6LL * a.mx + 6LL * ny);
+    const dim3 tpb(256, 1, 1);
+    const dim3 bpg((unsigned)((a.halo_count + 255) / 256), (unsigned)nfields, 1);
+    hipLaunchKernelGGL(periodic_fill_kernel<T>, bpg, tpb, 0, s, a);
+}
+
+// ---------------------------------------------------------------- reductions
+__device__ __forceinline__ double red_init(int op)
+{
+    return op == 1 ? INFINITY : (op == 2 ? -INFINITY : 0.0);
+}
+__device__ __forceinline__ double red_combine(int op, double a, double b)
+{
+    return op == 1 ? fmin(a, b) : (op == 2 ? fmax(a, b) : a + b);
+}
+
+template <typename T>
+__device__ __forceinline__ double red_map(int map, const T* const* bufs, long long c)
+{
+    switch (map) {
+    case 1: return (double)bufs[0][c];
+    case 2: { const double v = bufs[0][c]; return v * v; }
+    case 3: { const double e = exp((double)bufs[0][c]); return e * e; }
+    case 4: {
+        const double x = bufs[0][c], y = bufs[1][c], z = bufs[2][c];
+        return sqrt(x * x + y * y + z * z);
+    }
+    case 5: {
+        const double x = bufs[0][c], y = bufs[1][c], z = bufs[2][c];
+        return x * x + y * y + z * z;
+    }
+    case 6: {
+        const double x = exp((double)bufs[0][c]), y = exp((double)bufs[1][c]),
+                     z = exp((double)bufs[2][c]);
+        return x * x + y * y + z * z;
+    }
+    case 7: {
+        const double x = bufs[0][c], y = bufs[1][c], z = bufs[2][c];
+        return sqrt(x * x + y * y + z * z) / sqrt(exp((double)bufs[3][c]));
+    }
+    case 8: {
+        const double x = bufs[0][c], y = bufs[1][c], z = bufs[2][c];
+        return (x * x + y * y + z * z) / exp((double)bufs[3][c]);
+    }
+    default: return 0.0;
+    }
+}
+
+template <typename T>
+struct RedArgs {
+    const T* bufs[4];
+    int map, op;
+    int nx, ny, nz, mx, my;
+    double* partials;
+};
+
+__device__ __forceinline__ double block_reduce(int op, double v, double* sh)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        v = red_combine(op, v, __shfl_xor(v, off, 64));
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+        sh[wid] = v;
+    __syncthreads();
+    double r = red_init(op);
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int q = 0; q < nw; ++q)
+            r = red_combine(op, r, sh[q]);
+    }
+    return r;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(RED_TPB) reduce_stage1(RedArgs<T> a)
+{
+    __shared__ double sh[RED_TPB / 64];
+    const long long n = (long long)a.nx * a.ny * a.nz;
+    double acc = red_init(a.op);
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % a.nx);
+        const long long r = t / a.nx;
+        const int j = (int)(r % a.ny);
+        const int k = (int)(r / a.ny);
+        const long long c = (long long)(i + R) + (long long)a.mx * (j + R) +
+                            (long long)a.mx * a.my * (k + R);
+        acc = red_combine(a.op, acc, red_map(a.map, a.bufs, c));
+    }
+    const double r = block_reduce(a.op, acc, sh);
+    if (threadIdx.x == 0)
+        a.partials[blockIdx.x] = r;
+}
+
+__global__ void __launch_bounds__(RED_TPB) reduce_stage2(const double* partials, int np, int op,
+                                                         double* out)
+{
+    __shared__ double sh[RED_TPB / 64];
+    double acc = red_init(op);
+    for (int t = threadIdx.x; t < np; t += blockDim.x)
+        acc = red_combine(op, acc, partials[t]);
+    const double r = block_reduce(op, acc, sh);
+    if (threadIdx.x == 0)
+        *out = r;
+}
+
+int reduce_num_partials(int nx, int ny, int nz)
+{
+    const long long n = (long long)nx * ny * nz;
+    long long b = (n + RED_TPB - 1) / RED_TPB;
+    if (b > RED_MAX_BLOCKS)
+        b = RED_MAX_BLOCKS;
+    if (b < 1)
+        b = 1;
+    return (int)b;
+}
+
+template <typename T>
+void launch_reduce(const T* const* bufs, int map, int op, int nx, int ny, int nz, double* partials,
+                   int max_partials, double* result_dev, hipStream_t s)
+{
+    RedArgs<T> a;
+    for (int q = 0; q < 4; ++q)
+        a.bufs[q] = bufs[q];
+    a.map = map;
+    a.op = op;
+    a.nx = nx;
+    a.ny = ny;
+    a.nz = nz;
+    a.mx = nx + 2 * R;
+    a.my = ny + 2 * R;
+    a.partials = partials;
+    int nb = reduce_num_partials(nx, ny, nz);
+    if (nb > max_partials)
+        nb = max_partials;
+    hipLaunchKernelGGL(reduce_stage1<T>, dim3(nb), dim3(RED_TPB), 0, s, a);
+    hipLaunchKernelGGL(reduce_stage2, dim3(1), dim3(RED_TPB), 0, s, (const double*)partials, nb,
+                       op, result_dev);
+}
+
+// ---------------------------------------------------------------- diagnostics
+constexpr int DIAG_SLOTS = 4 * NF + 4;
+
+template <typename T>
+struct DiagArgs {
+    const T* f[NF];
+    Coeffs<T> C;
+    int nx, ny, nz, mx, my;
+    double* partials;  // DIAG_SLOTS * gridDim.x
+};
+
+__device__ __forceinline__ int diag_op(int slot)
+{
+    // [min x8][max x8][sum sq x8][sum u2][max u2][sum b2][max b2]
+    if (slot < NF)
+        return 1;
+    if (slot < 2 * NF)
+        return 2;
+    if (slot < 3 * NF)
+        return 3;
+    const int r = slot - 3 * NF;
+    return (r & 1) ? 2 : 3;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(RED_TPB) diagnostics_stage1(DiagArgs<T> a)
+{
+    __shared__ double sh[RED_TPB / 64];
+    double acc[DIAG_SLOTS];
+#pragma unroll
+    for (int q = 0; q < DIAG_SLOTS; ++q)
+        acc[q] = red_init(diag_op(q));
+    const long long n = (long long)a.nx * a.ny * a.nz;
+    const long long sy = a.mx, sz = (long long)a.mx * a.my;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % a.nx);
+        const long long r = t / a.nx;
+        const int j = (int)(r % a.ny);
+        const int k = (int)(r / a.ny);
+        const long long c = (long long)(i + R) + sy * (j + R) + sz * (k + R);
+#pragma unroll
+        for (int q = 0; q < NF; ++q) {
+            const double v = a.f[q][c];
+            acc[q] = fmin(acc[q], v);
+            acc[NF + q] = fmax(acc[NF + q], v);
+            acc[2 * NF + q] += v * v;
+        }
+        const double ux = a.f[1][c], uy = a.f[2][c], uz = a.f[3][c];
+        const double u2 = ux * ux + uy * uy + uz * uz;
+        acc[3 * NF + 0] += u2;
+        acc[3 * NF + 1] = fmax(acc[3 * NF + 1], u2);
+        const long long st[3] = {1, sy, sz};
+        double g[3][3];
+#pragma unroll
+        for (int comp = 0; comp < 3; ++comp) {
+            const T* f = a.f[4 + comp];
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) {
+                const long long s = st[ax];
+                const T* kk = a.C.d1[ax];
+                g[comp][ax] = (double)(kk[1] * (f[c + s] - f[c - s]) + kk[2] * (f[c + 2 * s] - f[c - 2 * s]) +
+                                       kk[3] * (f[c + 3 * s] - f[c - 3 * s]));
+            }
+        }
+        const double bx = g[2][1] - g[1][2], by = g[0][2] - g[2][0], bz = g[1][0] - g[0][1];
+        const double b2 = bx * bx + by * by + bz * bz;
+        acc[3 * NF + 2] += b2;
+        acc[3 * NF + 3] = fmax(acc[3 * NF + 3], b2);
+    }
+    for (int q = 0; q < DIAG_SLOTS; ++q) {
+        const double r = block_reduce(diag_op(q), acc[q], sh);
+        if (threadIdx.x == 0)
+            a.partials[(long long)q * gridDim.x + blockIdx.x] = r;
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(RED_TPB) diagnostics_stage2(const double* partials, int np, double* out)
+{
+    __shared__ double sh[RED_TPB / 64];
+    for (int q = 0; q < DIAG_SLOTS; ++q) {
+        const int op = diag_op(q);
+        double acc = red_init(op);
+        for (int t = threadIdx.x; t < np; t += blockDim.x)
+            acc = red_combine(op, acc, partials[(long long)q * np + t]);
+        const double r = block_reduce(op, acc, sh);
+        if (threadIdx.x == 0)
+            out[q] = r;
+        __syncthreads();
+    }
+}
+
+template <typename T>
+void launch_diagnostics(const T* const* fields, const Coeffs<T>& C, int nx, int ny, int nz,
+                        double* partials, int max_partials, double* out_dev, hipStream_t s)
+{
+    DiagArgs<T> a;
+    for (int q = 0; q < NF; ++q)
+        a.f[q] = fields[q];
+    a.C = C;
+    a.nx = nx;
+    a.ny = ny;
+    a.nz = nz;
+    a.mx = nx + 2 * R;
+    a.my = ny + 2 * R;
+    a.partials = partials;
+    int nb = reduce_num_partials(nx, ny, nz);
+    if (nb * DIAG_SLOTS > max_partials)
+        nb = max_partials / DIAG_SLOTS;
+    hipLaunchKernelGGL(diagnostics_stage1<T>, dim3(nb), dim3(RED_TPB), 0, s, a);
+    hipLaunchKernelGGL(diagnostics_stage2, dim3(1), dim3(RED_TPB), 0, s, (const double*)partials,
+                       nb, out_dev);
+}
+
+// ---------------------------------------------------------------- max |u|^2
+template <typename T>
+struct UmaxArgs {
+    const T* u[3];
+    int nx, ny, nz, mx, my;
+    unsigned long long* umax2;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(RED_TPB) umax2_kernel(UmaxArgs<T> a)
+{
+    const long long n = (long long)a.nx * a.ny * a.nz;
+    double m = 0.0;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % a.nx);
+        const long long r = t / a.nx;
+        const int j = (int)(r % a.ny);
+        const int k = (int)(r / a.ny);
+        const long long c = (long long)(i + R) + (long long)a.mx * (j + R) +
+                            (long long)a.mx * a.my * (k + R);
+        const double x = a.u[0][c], y = a.u[1][c], z = a.u[2][c];
+        m = fmax(m, x * x + y * y + z * z);
+    }
+    wave_max_to_global(m, a.umax2);
+}
+
+template <typename T>
+void launch_umax2(const T* const* uu, int nx, int ny, int nz, unsigned long long* umax2,
+                  hipStream_t s)
+{
+    UmaxArgs<T> a;
+    for (int q = 0; q < 3; ++q)
+        a.u[q] = uu[q];
+    a.nx = nx;
+    a.ny = ny;
+    a.nz = nz;
+    a.mx = nx + 2 * R;
+    a.my = ny + 2 * R;
+    a.umax2 = umax2;
+    const int nb = reduce_num_partials(nx, ny, nz);
+    hipLaunchKernelGGL(umax2_kernel<T>, dim3(nb), dim3(RED_TPB), 0, s, a);
+}
+
+// ---------------------------------------------------------------- fills
+template <typename T>
+__global__ void fill_kernel(T* dst, size_t n, T v)
+{
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (size_t)gridDim.x * blockDim.x)
+        dst[t] = v;
+}
+
+template <typename T>
+void launch_fill(T* dst, size_t n, T v, hipStream_t s)
+{
+    if (n == 0)
+        return;
+    size_t nb = (n + 255) / 256;
+    if (nb > 65536)
+        nb = 65536;
+    hipLaunchKernelGGL(fill_kernel<T>, dim3((unsigned)nb), dim3(256), 0, s, dst, n, v);
+}
+
+void launch_zero_bytes(void* dst, size_t nbytes, hipStream_t s)
+{
+    // Word fill for the aligned body, byte fill for a tail (slabs are 256-byte aligned).
+    const size_t words = nbytes / 4;
+    launch_fill<unsigned>((unsigned*)dst, words, 0u, s);
+    const size_t tail = nbytes - words * 4;
+    if (tail)
+        launch_fill<unsigned char>((unsigned char*)dst + words * 4, tail, (unsigned char)0, s);
+}
+
+template void launch_periodic_fill<AcRealT>(AcRealT* const*, int, int, int, int, hipStream_t);
+template void launch_reduce<AcRealT>(const AcRealT* const*, int, int, int, int, int, double*, int,
+                                     double*, hipStream_t);
+template void launch_diagnostics<AcRealT>(const AcRealT* const*, const Coeffs<AcRealT>&, int, int,
+                                          int, double*, int, double*, hipStream_t);
+template void launch_umax2<AcRealT>(const AcRealT* const*, int, int, int, unsigned long long*,
+                                    hipStream_t);
+template void launch_fill<AcRealT>(AcRealT*, size_t, AcRealT, hipStream_t);
+
+}  // namespace acamd

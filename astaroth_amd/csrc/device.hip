@@ -1,0 +1,1050 @@
+// device.hip -- the device layer of the C API (ref src/core/device.cc, include/astaroth_device.h)
+// on HIP for gfx950.
+//
+// Memory: one slab per role (in, out, w, and the optional Kahan compensation), each holding
+// NUM_VTXBUF_HANDLES contiguous m^3 arrays in the reference's x-fastest layout
+// (ref acc-runtime/api/vba.cc:313-470 allocates the in/out slabs the same way).
+// acDeviceSwapBuffer(s) swaps the in/out pointers of fields; w and the compensation stay put
+// because they are updated in place, point by point.
+//
+// Error behaviour: the reference aborts (ERRCHK_ALWAYS) on invariant violations such as a
+// device id out of range, a host/device size mismatch or a partial integration volume. Here the
+// same conditions print the reason to stderr and return AC_FAILURE, so a caller (or a test) can
+// recover; AC_NOT_ALLOCATED keeps its meaning for absent host buffers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/astaroth.h"
+#include "kernels.h"
+
+using namespace acamd;
+
+#define AC_HIP_CHECK(call)                                                                    \
+    do {                                                                                      \
+        const hipError_t err_ = (call);                                                       \
+        if (err_ != hipSuccess) {                                                             \
+            fprintf(stderr, "astaroth_amd: HIP error %s at %s:%d: %s\n", hipGetErrorName(err_), \
+                    __FILE__, __LINE__, #call);                                               \
+            return AC_FAILURE;                                                                \
+        }                                                                                     \
+    } while (0)
+
+static AcResult fail(const char* msg)
+{
+    fprintf(stderr, "astaroth_amd: %s\n", msg);
+    return AC_FAILURE;
+}
+
+namespace {
+constexpr int kTimingSlots = 4096;
+constexpr int kDiagSlots = 4 * NF + 4;
+}  // namespace
+
+struct device_s {
+    int id;
+    AcMeshInfo local_config;
+    hipStream_t streams[NUM_STREAMS];
+    bool stream_created[NUM_STREAMS];
+    // field buffers
+    AcReal* in[NF];
+    AcReal* out[NF];
+    AcReal* w[NF];
+    void* comp[NF];
+    void* slab_in;
+    void* slab_out;
+    void* slab_w;
+    void* slab_comp;
+    int kahan_bits;
+    size_t count;  // m^3
+    // reductions and device-resident scalars
+    double* partials;
+    int max_partials;
+    double* dres;                 // [0] reduce result, [1..] diagnostics
+    unsigned long long* umax2;    // two slots
+    double* time_acc;
+    int cfl_cur;
+    double cfl_courant;
+    double* host_pinned;          // staging for small D2H reads
+    // launch shape
+    TiledShape shape;
+    bool shape_tuned;
+    // timing
+    bool timing;
+    int timing_used;
+    hipEvent_t ev_start[kTimingSlots];
+    hipEvent_t ev_stop[kTimingSlots];
+    bool events_created;
+};
+
+static hipStream_t get_stream(Device d, Stream s)
+{
+    const int i = (s == STREAM_ALL || s < 0 || s >= NUM_STREAMS) ? 0 : (int)s;
+    if (!d->stream_created[i]) {
+        // Default priority only: streams of differing priority are not used anywhere.
+        if (hipStreamCreateWithFlags(&d->streams[i], hipStreamNonBlocking) != hipSuccess)
+            return nullptr;
+        d->stream_created[i] = true;
+    }
+    return d->streams[i];
+}
+
+static bool volume_eq(const Volume a, const int3 b)
+{
+    return a.x == (size_t)b.x && a.y == (size_t)b.y && a.z == (size_t)b.z;
+}
+
+static void make_coeffs(const AcMeshInfo& info, Coeffs<AcReal>& C)
+{
+    const double c1[4] = {0.0, 3.0 / 4.0, -3.0 / 20.0, 1.0 / 60.0};
+    const double c2[4] = {-49.0 / 18.0, 3.0 / 2.0, -3.0 / 20.0, 1.0 / 90.0};
+    const double cu[4] = {-20.0 / 60.0, 15.0 / 60.0, -6.0 / 60.0, 1.0 / 60.0};
+    const double cm[4] = {0.0, 270.0 / 720.0, -27.0 / 720.0, 2.0 / 720.0};
+    const AcReal3 ds = info.real3_params[AC_ds];
+    const double inv[3] = {1.0 / (double)ds.x, 1.0 / (double)ds.y, 1.0 / (double)ds.z};
+    for (int a = 0; a < 3; ++a)
+        for (int o = 0; o < 4; ++o) {
+            C.d1[a][o] = (AcReal)(c1[o] * inv[a]);
+            C.d2[a][o] = (AcReal)(c2[o] * inv[a] * inv[a]);
+            C.up[a][o] = (AcReal)(cu[o] * inv[a]);
+        }
+    const int pairs[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+    for (int p = 0; p < 3; ++p)
+        for (int o = 0; o < 4; ++o)
+            C.mx[p][o] = (AcReal)(cm[o] * inv[pairs[p][0]] * inv[pairs[p][1]]);
+}
+
+static void make_phys(const AcMeshInfo& info, PhysParams<AcReal>& P)
+{
+    const double cs = info.real_params[AC_cs_sound];
+    const double cp = info.real_params[AC_cp_sound];
+    const double gamma = info.real_params[AC_gamma];
+    const double mu0 = info.real_params[AC_mu0];
+    P.cs2_sound = (AcReal)(cs * cs);
+    P.nu = info.real_params[AC_nu_visc];
+    P.zeta = info.real_params[AC_zeta];
+    P.eta = info.real_params[AC_eta];
+    P.mu0 = (AcReal)mu0;
+    P.inv_mu0 = (AcReal)(1.0 / mu0);
+    P.cp = (AcReal)cp;
+    P.inv_cp = (AcReal)(1.0 / cp);
+    P.gamma = (AcReal)gamma;
+    P.gm1 = (AcReal)(gamma - 1.0);
+    P.lnT0 = info.real_params[AC_lnT0];
+    P.lnrho0 = info.real_params[AC_lnrho0];
+    P.K = info.real_params[AC_K_heatcond];
+    P.gamma_inv_cp = (AcReal)(gamma / cp);
+    P.eta_mu0 = (AcReal)((double)info.real_params[AC_eta] * mu0);
+    P.nu_third = (AcReal)((double)info.real_params[AC_nu_visc] / 3.0);
+}
+
+static void fill_solve_args(Device d, SolveArgs<AcReal>& a, int step, AcReal dt)
+{
+    const AcMeshInfo& info = d->local_config;
+    const int3 n = info.int3_params[AC_nlocal];
+    for (int q = 0; q < NF; ++q) {
+        a.in[q] = d->in[q];
+        a.out[q] = d->out[q];
+        a.w[q] = d->w[q];
+        a.comp[q] = d->comp[q];
+    }
+    a.nx = n.x;
+    a.ny = n.y;
+    a.nz = n.z;
+    a.mx = n.x + 2 * R;
+    a.my = n.y + 2 * R;
+    a.mz = n.z + 2 * R;
+    a.sz = (long long)a.mx * a.my;
+    make_coeffs(info, a.C);
+    make_phys(info, a.P);
+    const AcReal alpha[3] = {(AcReal)0.0, (AcReal)(-5.0 / 9.0), (AcReal)(-153.0 / 128.0)};
+    const AcReal beta[3] = {(AcReal)(1.0 / 3.0), (AcReal)(15.0 / 16.0), (AcReal)(8.0 / 15.0)};
+    a.dt = dt;
+    a.alpha = alpha[step];
+    a.beta = beta[step];
+    a.step = step;
+    a.umax2_cur = nullptr;
+    a.umax2_next = nullptr;
+    a.time_acc = nullptr;
+    a.cfl_cdt = 0.0;
+    a.cfl_dsmin = (double)info.real_params[AC_dsmin];
+    a.cfl_cs = (double)info.real_params[AC_cs_sound];
+}
+
+static AcResult launch_solve(Device d, const SolveArgs<AcReal>& a, hipStream_t s)
+{
+    const AcMeshInfo& info = d->local_config;
+    const bool upwd = info.bool_params[AC_lupwd] != 0;
+    const bool entropy = info.bool_params[AC_lentropy] != 0;
+    const int kb = d->kahan_bits;
+    const bool timed = d->timing && d->events_created && d->timing_used < kTimingSlots;
+    if (timed)
+        AC_HIP_CHECK(hipEventRecord(d->ev_start[d->timing_used], s));
+    bool done = false;
+    if (d->shape.tx > 0)
+        done = launch_solve_tiled<AcReal>(a, upwd, entropy, kb, d->shape, s);
+    if (!done)
+        launch_solve_direct<AcReal>(a, upwd, entropy, kb, s);
+    AC_HIP_CHECK(hipGetLastError());
+    if (timed) {
+        AC_HIP_CHECK(hipEventRecord(d->ev_stop[d->timing_used], s));
+        ++d->timing_used;
+    }
+    return AC_SUCCESS;
+}
+
+extern "C" {
+
+AcResult acDeviceCreate(const int id, const AcMeshInfo device_config, Device* device_handle)
+{
+    *device_handle = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+        return fail("acDeviceCreate: no HIP device available");
+    if (id < 0 || id >= count)
+        return fail("acDeviceCreate: device id out of range");
+    AC_HIP_CHECK(hipSetDevice(id));
+
+    device_s* d = new device_s();
+    memset((void*)d, 0, sizeof(*d));
+    d->id = id;
+    d->local_config = device_config;
+    int3& off = d->local_config.int3_params[AC_multigpu_offset];
+    if (off.x < 0 || off.y < 0 || off.z < 0) {
+        fprintf(stderr, "astaroth_amd: invalid AC_multigpu_offset, using (0,0,0)\n");
+        off.x = off.y = off.z = 0;
+    }
+    if (acHostUpdateParams(&d->local_config) != AC_SUCCESS) {
+        delete d;
+        return fail("acDeviceCreate: invalid grid configuration (AC_ngrid / AC_ds)");
+    }
+    const int3 m = d->local_config.int3_params[AC_mlocal];
+    d->count = (size_t)m.x * (size_t)m.y * (size_t)m.z;
+    int kb = d->local_config.int_params[AC_kahan_bits];
+    if (sizeof(AcReal) == 8)
+        kb = 0;
+    if (kb != 0 && kb != 8 && kb != 16 && kb != 32) {
+        delete d;
+        return fail("acDeviceCreate: AC_kahan_bits must be 0, 8, 16 or 32");
+    }
+    d->kahan_bits = kb;
+
+    hipStream_t s0 = get_stream(d, STREAM_0);
+    if (!s0) {
+        delete d;
+        return fail("acDeviceCreate: stream creation failed");
+    }
+    const size_t bytes = d->count * sizeof(AcReal) * NF;
+    if (hipMalloc(&d->slab_in, bytes) != hipSuccess || hipMalloc(&d->slab_out, bytes) != hipSuccess ||
+        hipMalloc(&d->slab_w, bytes) != hipSuccess) {
+        hipFree(d->slab_in);
+        hipFree(d->slab_out);
+        hipFree(d->slab_w);
+        delete d;
+        return fail("acDeviceCreate: device allocation failed");
+    }
+    size_t comp_bytes = 0;
+    if (kb > 0) {
+        comp_bytes = d->count * (size_t)(kb / 8) * NF;
+        if (hipMalloc(&d->slab_comp, comp_bytes) != hipSuccess) {
+            hipFree(d->slab_in);
+            hipFree(d->slab_out);
+            hipFree(d->slab_w);
+            delete d;
+            return fail("acDeviceCreate: device allocation of the compensation slab failed");
+        }
+    }
+    for (int q = 0; q < NF; ++q) {
+        d->in[q] = (AcReal*)d->slab_in + (size_t)q * d->count;
+        d->out[q] = (AcReal*)d->slab_out + (size_t)q * d->count;
+        d->w[q] = (AcReal*)d->slab_w + (size_t)q * d->count;
+        d->comp[q] = kb > 0 ? (void*)((char*)d->slab_comp + (size_t)q * d->count * (kb / 8)) : nullptr;
+    }
+    launch_zero_bytes(d->slab_in, bytes, s0);
+    launch_zero_bytes(d->slab_out, bytes, s0);
+    launch_zero_bytes(d->slab_w, bytes, s0);
+    if (kb > 0)
+        launch_zero_bytes(d->slab_comp, comp_bytes, s0);
+
+    const int3 n = d->local_config.int3_params[AC_nlocal];
+    d->max_partials = std::max(reduce_num_partials(n.x, n.y, n.z), 1) * kDiagSlots;
+    AC_HIP_CHECK(hipMalloc(&d->partials, sizeof(double) * d->max_partials));
+    AC_HIP_CHECK(hipMalloc(&d->dres, sizeof(double) * (1 + kDiagSlots)));
+    AC_HIP_CHECK(hipMalloc(&d->umax2, sizeof(unsigned long long) * 2));
+    AC_HIP_CHECK(hipMalloc(&d->time_acc, sizeof(double)));
+    launch_zero_bytes(d->umax2, sizeof(unsigned long long) * 2, s0);
+    launch_zero_bytes(d->time_acc, sizeof(double), s0);
+    AC_HIP_CHECK(hipHostMalloc((void**)&d->host_pinned, sizeof(double) * (2 + kDiagSlots)));
+
+    d->shape.tx = d->local_config.int_params[AC_solve_tile_x];
+    d->shape.ty = d->local_config.int_params[AC_solve_tile_y];
+    d->shape.zchunk = d->local_config.int_params[AC_solve_zchunk];
+    if (d->shape.tx > 0 && !tiled_shape_fits(d->shape, n.x, n.y, n.z))
+        d->shape.tx = 0;
+    d->shape_tuned = d->shape.tx > 0;
+    AC_HIP_CHECK(hipGetLastError());
+    AC_HIP_CHECK(hipStreamSynchronize(s0));
+    *device_handle = d;
+    return AC_SUCCESS;
+}
+
+AcResult acDeviceDestroy(Device* device_ptr)
+{
+    Device d = *device_ptr;
+    if (!d)
+        return AC_SUCCESS;
+    hipSetDevice(d->id);
+    hipDeviceSynchronize();
+    hipFree(d->slab_in);
+    hipFree(d->slab_out);
+    hipFree(d->slab_w);
+    if (d->slab_comp)
+        hipFree(d->slab_comp);
+    hipFree(d->partials);
+    hipFree(d->dres);
+    hipFree(d->umax2);
+    hipFree(d->time_acc);
+    hipHostFree(d->host_pinned);
+    if (d->events_created)
+        for (int i = 0; i < kTimingSlots; ++i) {
+            hipEventDestroy(d->ev_start[i]);
+            hipEventDestroy(d->ev_stop[i]);
+        }
+    for (int i = 0; i < NUM_STREAMS; ++i)
+        if (d->stream_created[i])
+            hipStreamDestroy(d->streams[i]);
+    delete d;
+    *device_ptr = nullptr;
+    return AC_SUCCESS;
+}
+
+AcResult acDeviceResetMesh(const Device d, const Stream stream)
+{
+    hipSetDevice(d->id);
+    hipStream_t s = get_stream(d, stream);
+    const size_t bytes = d->count * sizeof(AcReal) * NF;
+    launch_zero_bytes(d->slab_in, bytes, s);
+    launch_zero_bytes(d->slab_out, bytes, s);
+    launch_zero_bytes(d->slab_w, bytes, s);
+    if (d->slab_comp)
+        launch_zero_bytes(d->slab_comp, d->count * (size_t)(d->kahan_bits / 8) * NF, s);
+    AC_HIP_CHECK(hipGetLastError());
+    return AC_SUCCESS;
+}
+
+AcResult acDevicePrintInfo(const Device d)
+{
+    hipDeviceProp_t p;
+    AC_HIP_CHECK(hipGetDeviceProperties(&p, d->id));
+    const int3 n = d->local_config.int3_params[AC_nlocal];
+    printf("Device %d: %s (%s), %d CUs, %.1f GiB\n", d->id, p.name, p.gcnArchName,
+           p.multiProcessorCount, p.totalGlobalMem / 1073741824.0);
+    printf("  grid %d x %d x %d, AcReal %zu bytes, kahan bits %d, solve shape %d x %d x %d\n",
+           n.x, n.y, n.z, sizeof(AcReal), d->kahan_bits, d->shape.tx, d->shape.ty, d->shape.zchunk);
+    return AC_SUCCESS;
+}
+
+AcResult acDeviceSynchronizeStream(const Device d, const Stream stream)
+{
+    AC_HIP_CHECK(hipSetDevice(d->id));
+    if (stream == STREAM_ALL) {
+        AC_HIP_CHECK(hipDeviceSynchronize());
+    }
+    else {
+        AC_HIP_CHECK(hipStreamSynchronize(get_stream(d, stream)));
+    }
+    return AC_SUCCESS;
+}
+
+AcResult acDeviceSwapBuffer(const Device d, const VertexBufferHandle h)
+{
+    if ((int)h < 0 || (int)h >= NF)
+        return fail("acDeviceSwapBuffer: invalid handle");
+    std::swap(d->in[h], d->out[h]);
+    return AC_SUCCESS;
+}
+
+AcResult acDeviceSwapBuffers(const Device d)
+{
+    for (int q = 0; q < NF; ++q)
+        std::swap(d->in[q], d->out[q]);
+    return AC_SUCCESS;
+}
+
+AcResult acDeviceLoadScalarUniform(const Device d, const Stream, const AcRealParam param,
+                                   const AcReal value)
+{
+    if ((int)param < 0 || (int)param >= NUM_REAL_PARAMS)
+        return fail("acDeviceLoadScalarUniform: invalid parameter");
+    d->local_config.real_params[param] = value;
+    return acHostUpdateParams(&d->local_config);
+}
+
+AcResult acDeviceLoadVectorUniform(const Device d, const Stream, const AcReal3Param param,
+                                   const AcReal3 value)
+{
+    if ((int)param < 0 || (int)param >= NUM_REAL3_PARAMS)
+        return fail("acDeviceLoadVectorUniform: invalid parameter");
+    d->local_config.real3_params[param] = value;
+    return acHostUpdateParams(&d->local_config);
+}
+
+AcResult acDeviceLoadIntUniform(const Device d, const Stream, const AcIntParam param, const int value)
+{
+    if ((int)param < 0 || (int)param >= NUM_INT_PARAMS)
+        return fail("acDeviceLoadIntUniform: invalid parameter");
+    if (param == AC_kahan_bits && value != d->kahan_bits)
+        return fail("acDeviceLoadIntUniform: AC_kahan_bits is fixed at acDeviceCreate");
+    d->local_config.int_params[param] = value;
+    if (param == AC_solve_tile_x || param == AC_solve_tile_y || param == AC_solve_zchunk) {
+        const int3 n = d->local_config.int3_params[AC_nlocal];
+        TiledShape s{d->local_config.int_params[AC_solve_tile_x],
+                     d->local_config.int_params[AC_solve_tile_y],
+                     d->local_config.int_params[AC_solve_zchunk]};
+        if (s.tx > 0 && tiled_shape_fits(s, n.x, n.y, n.z)) {
+            d->shape = s;
+            d->shape_tuned = true;
+        }
+        else if (s.tx == 0) {
+            d->shape = TiledShape{0, 0, 0};
+            d->shape_tuned = false;
+        }
+        else {
+            return fail("acDeviceLoadIntUniform: solve launch shape does not fit the grid");
+        }
+    }
+    return AC_SUCCESS;
+}
+
+AcResult acDeviceLoadBoolUniform(const Device d, const Stream, const AcBoolParam param,
+                                 const bool value)
+{
+    if ((int)param < 0 || (int)param >= NUM_BOOL_PARAMS)
+        return fail("acDeviceLoadBoolUniform: invalid parameter");
+    d->local_config.bool_params[param] = value ? 1 : 0;
+    return AC_SUCCESS;
+}
+
+AcResult acDeviceLoadInt3Uniform(const Device d, const Stream, const AcInt3Param param,
+                                 const int3 value)
+{
+    if ((int)param < 0 || (int)param >= NUM_INT3_PARAMS)
+        return fail("acDeviceLoadInt3Uniform: invalid parameter");
+    if (param == AC_ngrid || param == AC_nlocal || param == AC_mlocal || param == AC_mgrid ||
+        param == AC_nmin || param == AC_nlocal_max || param == AC_ngrid_max)
+        return fail("acDeviceLoadInt3Uniform: grid dimensions are fixed at acDeviceCreate");
+    d->local_config.int3_params[param] = value;
+    return AC_SUCCESS;
+}
+
+AcResult acDeviceStoreScalarUniform(const Device d, const Stream, const AcRealParam param,
+                                    AcReal* value)
+{
+    if ((int)param < 0 || (int)param >= NUM_REAL_PARAMS)
+        return fail("acDeviceStoreScalarUniform: invalid parameter");
+    *value = d->local_config.real_params[param];
+    return AC_SUCCESS;
+}
+
+AcResult acDeviceStoreVectorUniform(const Device d, const Stream, const AcReal3Param param,
+                                    AcReal3* value)
+{
+    if ((int)param < 0 || (int)param >= NUM_REAL3_PARAMS)
+        return fail("acDeviceStoreVectorUniform: invalid parameter");
+    *value = d->local_config.real3_params[param];
+    return AC_SUCCESS;
+}
+
+AcResult acDeviceLoadMeshInfo(const Device d, const AcMeshInfo config)
+{
+    AcMeshInfo c = config;
+    if (acHostUpdateParams(&c) != AC_SUCCESS)
+        return fail("acDeviceLoadMeshInfo: invalid configuration");
+    for (int p = AC_ngrid; p <= AC_ngrid_max; ++p) {
+        const int3 a = c.int3_params[p], b = d->local_config.int3_params[p];
+        if (a.x != b.x || a.y != b.y || a.z != b.z)
+            return fail("acDeviceLoadMeshInfo: grid dimensions differ from acDeviceCreate");
+    }
+    c.int_params[AC_kahan_bits] = d->kahan_bits;
+    d->local_config = c;
+    return AC_SUCCESS;
+}
+
+AcMeshInfo acDeviceGetConfig(const Device d) { return d->local_config; }
+AcMeshInfo acDeviceGetLocalConfig(const Device d) { return d->local_config; }
+
+static size_t host_index(const int3 p, const AcMeshInfo& info)
+{
+    const int3 m = info.int3_params[AC_mlocal];
+    return (size_t)p.x + (size_t)m.x * ((size_t)p.y + (size_t)m.y * (size_t)p.z);
+}
+
+static size_t info_count(const AcMeshInfo& info)
+{
+    const int3 m = info.int3_params[AC_mlocal];
+    return (size_t)m.x * (size_t)m.y * (size_t)m.z;
+}
+
+AcResult acDeviceLoadVertexBufferWithOffset(const Device d, const Stream stream,
+                                            const AcMesh host_mesh, const VertexBufferHandle h,
+                                            const int3 src, const int3 dst, const int num_vertices)
+{
+    if ((int)h < 0 || (int)h >= NF)
+        return fail("acDeviceLoadVertexBufferWithOffset: invalid handle");
+    if (!host_mesh.vertex_buffer[h])
+        return AC_NOT_ALLOCATED;
+    AC_HIP_CHECK(hipSetDevice(d->id));
+    const size_t si = host_index(src, host_mesh.info);
+    const size_t di = host_index(dst, d->local_config);
+    if (num_vertices < 0 || di + (size_t)num_vertices > d->count ||
+        si + (size_t)num_vertices > info_count(host_mesh.info))
+        return fail("acDeviceLoadVertexBufferWithOffset: range outside the mesh");
+    hipStream_t s = get_stream(d, stream);
+    AC_HIP_CHECK(hipMemcpyAsync(d->in[h] + di, host_mesh.vertex_buffer[h] + si,
+                                sizeof(AcReal) * (size_t)num_vertices, hipMemcpyHostToDevice, s));
+    if (d->kahan_bits > 0)
+        launch_zero_bytes(d->comp[h], d->count * (size_t)(d->kahan_bits / 8), s);
+    return AC_SUCCESS;
+}
+
+AcResult acDeviceLoadMeshWithOffset(const Device d, const Stream stream, const AcMesh host_mesh,
+                                    const int3 src, const int3 dst, const int num_vertices)
+{
+    int r = 0;
+    for (int q = 0; q < NF; ++q)
+        r |= acDeviceLoadVertexBufferWithOffset(d, stream, host_mesh, (VertexBufferHandle)q, src,
+                                                dst, num_vertices);
+    return (AcResult)r;
+}
+
+AcResult acDeviceLoadVertexBuffer(const Device d, const Stream stream, const AcMesh host_mesh,
+                                  const VertexBufferHandle h)
+{
+    if (info_count(host_mesh.info) != d->count)
+        return fail("acDeviceLoadVertexBuffer: host and device mesh sizes differ");
+    int3 z;
+    z.x = z.y = z.z = 0;
+    return acDeviceLoadVertexBufferWithOffset(d, stream, host_mesh, h, z, z, (int)d->count);
+}
+
+AcResult acDeviceLoadMesh(const Device d, const Stream stream, const AcMesh host_mesh)
+{
+    int r = 0;
+    for (int q = 0; q < NF; ++q)
+        r |= acDeviceLoadVertexBuffer(d, stream, host_mesh, (VertexBufferHandle)q);
+    return (AcResult)r;
+}
+
+AcResult acDeviceSetVertexBuffer(const Device d, const Stream stream, const VertexBufferHandle h,
+                                 const AcReal value)
+{
+    if ((int)h < 0 || (int)h >= NF)
+        return fail("acDeviceSetVertexBuffer: invalid handle");
+    AC_HIP_CHECK(hipSetDevice(d->id));
+    hipStream_t s = get_stream(d, stream);
+    launch_fill<AcReal>(d->in[h], d->count, value, s);
+    launch_fill<AcReal>(d->out[h], d->count, value, s);
+    if (d->kahan_bits > 0)
+        launch_zero_bytes(d->comp[h], d->count * (size_t)(d->kahan_bits / 8), s);
+    AC_HIP_CHECK(hipGetLastError());
+    return AC_SUCCESS;
+}
+
+AcResult acDeviceFlushOutputBuffers(const Device d, const Stream stream)
+{
+    AC_HIP_CHECK(hipSetDevice(d->id));
+    hipStream_t s = get_stream(d, stream);
+    for (int q = 0; q < NF; ++q)
+        launch_fill<AcReal>(d->out[q], d->count, (AcReal)0, s);
+    AC_HIP_CHECK(hipGetLastError());
+    return AC_SUCCESS;
+}
+
+AcResult acDeviceStoreVertexBufferWithOffset(const Device d, const Stream stream,
+                                             const VertexBufferHandle h, const int3 src,
+                                             const int3 dst, const int num_vertices,
+                                             AcMesh* host_mesh)
+{
+    if ((int)h < 0 || (int)h >= NF)
+        return fail("acDeviceStoreVertexBufferWithOffset: invalid handle");
+    if (!host_mesh->vertex_buffer[h])
+        return AC_NOT_ALLOCATED;
+    AC_HIP_CHECK(hipSetDevice(d->id));
+    const size_t si = host_index(src, d->local_config);
+    const size_t di = host_index(dst, host_mesh->info);
+    if (num_vertices < 0 || si + (size_t)num_vertices > d->count ||
+        di + (size_t)num_vertices > info_count(host_mesh->info))
+        return fail("acDeviceStoreVertexBufferWithOffset: range outside the mesh");
+    hipStream_t s = get_stream(d, stream);
+    AC_HIP_CHECK(hipMemcpyAsync(host_mesh->vertex_buffer[h] + di, d->in[h] + si,
+                                sizeof(AcReal) * (size_t)num_vertices, hipMemcpyDeviceToHost, s));
+    return AC_SUCCESS;
+}
+
+AcResult acDeviceStoreMeshWithOffset(const Device d, const Stream stream, const int3 src,
+                                     const int3 dst, const int num_vertices, AcMesh* host_mesh)
+{
+    int r = 0;
+    for (int q = 0; q < NF; ++q)
+        r |= acDeviceStoreVertexBufferWithOffset(d, stream, (VertexBufferHandle)q, src, dst,
+                                                 num_vertices, host_mesh);
+    return (AcResult)r;
+}
+
+AcResult acDeviceStoreVertexBuffer(const Device d, const Stream stream, const VertexBufferHandle h,
+                                   AcMesh* host_mesh)
+{
+    if (info_count(host_mesh->info) != d->count)
+        return fail("acDeviceStoreVertexBuffer: host and device mesh sizes differ");
+    int3 z;
+    z.x = z.y = z.z = 0;
+    return acDeviceStoreVertexBufferWithOffset(d, stream, h, z, z, (int)d->count, host_mesh);
+}
+
+AcResult acDeviceStoreMesh(const Device d, const Stream stream, AcMesh* host_mesh)
+{
+    int r = 0;
+    for (int q = 0; q < NF; ++q)
+        r |= acDeviceStoreVertexBuffer(d, stream, (VertexBufferHandle)q, host_mesh);
+    return (AcResult)r;
+}
+
+static AcResult check_interior(Device d, const Volume start, const Volume end, const char* who)
+{
+    const AcMeshInfo& info = d->local_config;
+    if (!volume_eq(start, info.int3_params[AC_nmin]) || !volume_eq(end, info.int3_params[AC_nlocal_max])) {
+        fprintf(stderr, "astaroth_amd: %s: start/end must be the full interior [nmin, nlocal_max)\n", who);
+        return AC_FAILURE;
+    }
+    return AC_SUCCESS;
+}
+
+static AcResult ensure_shape(Device d, hipStream_t s);
+
+AcResult acDeviceIntegrateSubstep(const Device d, const Stream stream, const int step_number,
+                                  const Volume start, const Volume end, const AcReal dt)
+{
+    if (check_interior(d, start, end, "acDeviceIntegrateSubstep") != AC_SUCCESS)
+        return AC_FAILURE;
+    if (step_number < 0 || step_number > 2)
+        return fail("acDeviceIntegrateSubstep: step_number must be 0, 1 or 2");
+    AC_HIP_CHECK(hipSetDevice(d->id));
+    hipStream_t s = get_stream(d, stream);
+    if (ensure_shape(d, s) != AC_SUCCESS)
+        return AC_FAILURE;
+    SolveArgs<AcReal> a;
+    fill_solve_args(d, a, step_number, dt);
+    return launch_solve(d, a, s);
+}
+
+AcResult acDevicePeriodicBoundcondStep(const Device d, const Stream stream,
+                                       const VertexBufferHandle h, const Volume start,
+                                       const Volume end)
+{
+    if ((int)h < 0 || (int)h >= NF)
+        return fail("acDevicePeriodicBoundcondStep: invalid handle");
+    const int3 m = d->local_config.int3_params[AC_mlocal];
+    if (start.x != 0 || start.y != 0 || start.z != 0 || !volume_eq(end, m))
+        return fail("acDevicePeriodicBoundcondStep: only the full volume [0, mlocal) is supported");
+    AC_HIP_CHECK(hipSetDevice(d->id));
+    const int3 n = d->local_config.int3_params[AC_nlocal];
+    AcReal* f[1] = {d->in[h]};
+    launch_periodic_fill<AcReal>(f, 1, n.x, n.y, n.z, get_stream(d, stream));
+    AC_HIP_CHECK(hipGetLastError());
+    return AC_SUCCESS;
+}
+
+AcResult acDevicePeriodicBoundconds(const Device d, const Stream stream, const Volume start,
+                                    const Volume end)
+{
+    const int3 m = d->local_config.int3_params[AC_mlocal];
+    if (start.x != 0 || start.y != 0 || start.z != 0 || !volume_eq(end, m))
+        return fail("acDevicePeriodicBoundconds: only the full volume [0, mlocal) is supported");
+    AC_HIP_CHECK(hipSetDevice(d->id));
+    const int3 n = d->local_config.int3_params[AC_nlocal];
+    launch_periodic_fill<AcReal>(d->in, NF, n.x, n.y, n.z, get_stream(d, stream));
+    AC_HIP_CHECK(hipGetLastError());
+    return AC_SUCCESS;
+}
+
+static AcResult reduce_common(Device d, const Stream stream, const AcReduction r, int map,
+                              const VertexBufferHandle* h, int nh, bool post, AcReal* result)
+{
+    for (int q = 0; q < nh; ++q)
+        if ((int)h[q] < 0 || (int)h[q] >= NF)
+            return fail("acDeviceReduce: invalid handle");
+    if (map == AC_NULL_KERNEL || r.reduce_op == AC_REDUCE_OP_NO_REDUCE)
+        return fail("acDeviceReduce: reduction type not defined for this arity");
+    AC_HIP_CHECK(hipSetDevice(d->id));
+    hipStream_t s = get_stream(d, stream);
+    const AcReal* bufs[4] = {nullptr, nullptr, nullptr, nullptr};
+    for (int q = 0; q < nh; ++q)
+        bufs[q] = d->in[h[q]];
+    const int3 n = d->local_config.int3_params[AC_nlocal];
+    launch_reduce<AcReal>(bufs, (int)map, (int)r.reduce_op, n.x, n.y, n.z, d->partials,
+                          d->max_partials, d->dres, s);
+    AC_HIP_CHECK(hipGetLastError());
+    AC_HIP_CHECK(hipMemcpyAsync(d->host_pinned, d->dres, sizeof(double), hipMemcpyDeviceToHost, s));
+    AC_HIP_CHECK(hipStreamSynchronize(s));
+    double v = d->host_pinned[0];
+    if (post && r.post_processing_op == AC_RMS)
+        v = sqrt(v / ((double)n.x * n.y * n.z));
+    *result = (AcReal)v;
+    return AC_SUCCESS;
+}
+
+AcResult acDeviceReduceScalNoPostProcessing(const Device d, const Stream stream,
+                                            const AcReduction r, const VertexBufferHandle h,
+                                            AcReal* result)
+{
+    return reduce_common(d, stream, r, r.map_vtxbuf_single, &h, 1, false, result);
+}
+
+AcResult acDeviceReduceScal(const Device d, const Stream stream, const AcReduction r,
+                            const VertexBufferHandle h, AcReal* result)
+{
+    return reduce_common(d, stream, r, r.map_vtxbuf_single, &h, 1, true, result);
+}
+
+AcResult acDeviceReduceVecNoPostProcessing(const Device d, const Stream stream, const AcReduction r,
+                                           const VertexBufferHandle a, const VertexBufferHandle b,
+                                           const VertexBufferHandle c, AcReal* result)
+{
+    const VertexBufferHandle h[3] = {a, b, c};
+    return reduce_common(d, stream, r, r.map_vtxbuf_vec, h, 3, false, result);
+}
+
+AcResult acDeviceReduceVec(const Device d, const Stream stream, const AcReduction r,
+                           const VertexBufferHandle a, const VertexBufferHandle b,
+                           const VertexBufferHandle c, AcReal* result)
+{
+    const VertexBufferHandle h[3] = {a, b, c};
+    return reduce_common(d, stream, r, r.map_vtxbuf_vec, h, 3, true, result);
+}
+
+AcResult acDeviceReduceVecScalNoPostProcessing(const Device d, const Stream stream,
+                                               const AcReduction r, const VertexBufferHandle a,
+                                               const VertexBufferHandle b,
+                                               const VertexBufferHandle c,
+                                               const VertexBufferHandle e, AcReal* result)
+{
+    const VertexBufferHandle h[4] = {a, b, c, e};
+    return reduce_common(d, stream, r, r.map_vtxbuf_vec_scal, h, 4, false, result);
+}
+
+AcResult acDeviceReduceVecScal(const Device d, const Stream stream, const AcReduction r,
+                               const VertexBufferHandle a, const VertexBufferHandle b,
+                               const VertexBufferHandle c, const VertexBufferHandle e,
+                               AcReal* result)
+{
+    const VertexBufferHandle h[4] = {a, b, c, e};
+    return reduce_common(d, stream, r, r.map_vtxbuf_vec_scal, h, 4, true, result);
+}
+
+// ------------------------------------------------------------------ extensions
+
+AcResult acDeviceUpdateTimestepCFL(const Device d, const Stream stream, const AcReal courant)
+{
+    AC_HIP_CHECK(hipSetDevice(d->id));
+    hipStream_t s = get_stream(d, stream);
+    const int3 n = d->local_config.int3_params[AC_nlocal];
+    unsigned long long* slot = d->umax2 + d->cfl_cur;
+    launch_zero_bytes(slot, sizeof(unsigned long long), s);
+    const AcReal* uu[3] = {d->in[VTXBUF_UUX], d->in[VTXBUF_UUY], d->in[VTXBUF_UUZ]};
+    launch_umax2<AcReal>(uu, n.x, n.y, n.z, slot, s);
+    AC_HIP_CHECK(hipGetLastError());
+    d->cfl_courant = (double)courant;
+    return AC_SUCCESS;
+}
+
+AcResult acDeviceGetTimestep(const Device d, const Stream stream, double* dt)
+{
+    AC_HIP_CHECK(hipSetDevice(d->id));
+    hipStream_t s = get_stream(d, stream);
+    AC_HIP_CHECK(hipMemcpyAsync(d->host_pinned, d->umax2 + d->cfl_cur, sizeof(double),
+                                hipMemcpyDeviceToHost, s));
+    AC_HIP_CHECK(hipStreamSynchronize(s));
+    const double umax = sqrt(d->host_pinned[0]);
+    const AcMeshInfo& info = d->local_config;
+    *dt = d->cfl_courant * (double)info.real_params[AC_dsmin] /
+          ((double)info.real_params[AC_cs_sound] + umax);
+    return AC_SUCCESS;
+}
+
+AcResult acDeviceIntegrateSubstepDeviceDt(const Device d, const Stream stream,
+                                          const int step_number, const AcReal courant)
+{
+    if (step_number < 0 || step_number > 2)
+        return fail("acDeviceIntegrateSubstepDeviceDt: step_number must be 0, 1 or 2");
+    AC_HIP_CHECK(hipSetDevice(d->id));
+    hipStream_t s = get_stream(d, stream);
+    if (ensure_shape(d, s) != AC_SUCCESS)
+        return AC_FAILURE;
+    SolveArgs<AcReal> a;
+    fill_solve_args(d, a, step_number, (AcReal)0);
+    a.cfl_cdt = (double)courant;
+    a.umax2_cur = (const double*)(d->umax2 + d->cfl_cur);
+    const int nxt = 1 - d->cfl_cur;
+    if (step_number == 2) {
+        launch_zero_bytes(d->umax2 + nxt, sizeof(unsigned long long), s);
+        a.umax2_next = d->umax2 + nxt;
+        a.time_acc = d->time_acc;
+    }
+    const AcResult r = launch_solve(d, a, s);
+    if (step_number == 2)
+        d->cfl_cur = nxt;
+    d->cfl_courant = (double)courant;
+    return r;
+}
+
+AcResult acDeviceIntegrateStepsCFL(const Device d, const Stream stream, const int nsteps,
+                                   const AcReal courant, double* time_out)
+{
+    AC_HIP_CHECK(hipSetDevice(d->id));
+    hipStream_t s = get_stream(d, stream);
+    const int3 n = d->local_config.int3_params[AC_nlocal];
+    if (time_out)
+        launch_zero_bytes(d->time_acc, sizeof(double), s);
+    for (int step = 0; step < nsteps; ++step) {
+        for (int sub = 0; sub < 3; ++sub) {
+            launch_periodic_fill<AcReal>(d->in, NF, n.x, n.y, n.z, s);
+            if (acDeviceIntegrateSubstepDeviceDt(d, stream, sub, courant) != AC_SUCCESS)
+                return AC_FAILURE;
+            acDeviceSwapBuffers(d);
+        }
+    }
+    AC_HIP_CHECK(hipGetLastError());
+    if (time_out) {
+        AC_HIP_CHECK(hipMemcpyAsync(d->host_pinned, d->time_acc, sizeof(double), hipMemcpyDeviceToHost, s));
+        AC_HIP_CHECK(hipStreamSynchronize(s));
+        *time_out += d->host_pinned[0];
+    }
+    return AC_SUCCESS;
+}
+
+AcResult acDeviceReduceDiagnostics(const Device d, const Stream stream, AcDiagnostics* out)
+{
+    AC_HIP_CHECK(hipSetDevice(d->id));
+    hipStream_t s = get_stream(d, stream);
+    const int3 n = d->local_config.int3_params[AC_nlocal];
+    Coeffs<AcReal> C;
+    make_coeffs(d->local_config, C);
+    const AcReal* f[NF];
+    for (int q = 0; q < NF; ++q)
+        f[q] = d->in[q];
+    launch_diagnostics<AcReal>(f, C, n.x, n.y, n.z, d->partials, d->max_partials, d->dres + 1, s);
+    AC_HIP_CHECK(hipGetLastError());
+    AC_HIP_CHECK(hipMemcpyAsync(d->host_pinned, d->dres + 1, sizeof(double) * kDiagSlots,
+                                hipMemcpyDeviceToHost, s));
+    AC_HIP_CHECK(hipStreamSynchronize(s));
+    const double* v = d->host_pinned;
+    const double inv_n = 1.0 / ((double)n.x * n.y * n.z);
+    for (int q = 0; q < NF; ++q) {
+        out->min[q] = v[q];
+        out->max[q] = v[NF + q];
+        out->rms[q] = sqrt(v[2 * NF + q] * inv_n);
+    }
+    out->uu_rms = sqrt(v[3 * NF + 0] * inv_n);
+    out->uu_max = sqrt(v[3 * NF + 1]);
+    out->bb_rms = sqrt(v[3 * NF + 2] * inv_n);
+    out->bb_max = sqrt(v[3 * NF + 3]);
+    return AC_SUCCESS;
+}
+
+AcResult acDeviceGetVertexBufferPtrs(const Device d, const VertexBufferHandle h, AcReal** in,
+                                     AcReal** out, AcReal** w)
+{
+    if ((int)h < 0 || (int)h >= NF)
+        return fail("acDeviceGetVertexBufferPtrs: invalid handle");
+    if (in)
+        *in = d->in[h];
+    if (out)
+        *out = d->out[h];
+    if (w)
+        *w = d->w[h];
+    return AC_SUCCESS;
+}
+
+AcResult acDeviceSetSolveTiming(const Device d, const int enable)
+{
+    AC_HIP_CHECK(hipSetDevice(d->id));
+    if (enable && !d->events_created) {
+        for (int i = 0; i < kTimingSlots; ++i) {
+            AC_HIP_CHECK(hipEventCreate(&d->ev_start[i]));
+            AC_HIP_CHECK(hipEventCreate(&d->ev_stop[i]));
+        }
+        d->events_created = true;
+    }
+    d->timing = enable != 0;
+    d->timing_used = 0;
+    return AC_SUCCESS;
+}
+
+AcResult acDeviceGetSolveTiming(const Device d, double* total_ms, int* launches)
+{
+    AC_HIP_CHECK(hipSetDevice(d->id));
+    AC_HIP_CHECK(hipDeviceSynchronize());
+    double tot = 0.0;
+    for (int i = 0; i < d->timing_used; ++i) {
+        float ms = 0.f;
+        AC_HIP_CHECK(hipEventElapsedTime(&ms, d->ev_start[i], d->ev_stop[i]));
+        tot += ms;
+    }
+    if (total_ms)
+        *total_ms = tot;
+    if (launches)
+        *launches = d->timing_used;
+    return AC_SUCCESS;
+}
+
+AcResult acDeviceGetStreamHandle(const Device d, const Stream stream, void** hip_stream)
+{
+    AC_HIP_CHECK(hipSetDevice(d->id));
+    hipStream_t s = get_stream(d, stream);
+    if (!s)
+        return fail("acDeviceGetStreamHandle: stream creation failed");
+    *hip_stream = (void*)s;
+    return AC_SUCCESS;
+}
+
+AcResult acDeviceGetSolveLaunchShape(const Device d, int* tile_x, int* tile_y, int* zchunk)
+{
+    *tile_x = d->shape.tx;
+    *tile_y = d->shape.ty;
+    *zchunk = d->shape.zchunk;
+    return AC_SUCCESS;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ autotuner
+// Times every candidate shape of the tiled kernel on the current grid (dry run into the out and
+// w buffers from a copy of the state, so the caller's data is untouched) and keeps the fastest.
+// Candidates that do not fit the grid are skipped. The choice can be cached in the file named
+// by ASTAROTH_AUTOTUNE_CACHE (lines "nx ny nz kahan upwd entropy tx ty zchunk").
+static bool read_cache(Device d, TiledShape* out)
+{
+    const char* path = getenv("ASTAROTH_AUTOTUNE_CACHE");
+    if (!path)
+        return false;
+    FILE* fp = fopen(path, "r");
+    if (!fp)
+        return false;
+    const int3 n = d->local_config.int3_params[AC_nlocal];
+    int v[9];
+    bool found = false;
+    while (fscanf(fp, "%d %d %d %d %d %d %d %d %d", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6],
+                  &v[7], &v[8]) == 9) {
+        if (v[0] == n.x && v[1] == n.y && v[2] == n.z && v[3] == d->kahan_bits &&
+            v[4] == d->local_config.bool_params[AC_lupwd] &&
+            v[5] == d->local_config.bool_params[AC_lentropy]) {
+            TiledShape s{v[6], v[7], v[8]};
+            if (s.tx == 0 || tiled_shape_fits(s, n.x, n.y, n.z)) {
+                *out = s;
+                found = true;
+            }
+        }
+    }
+    fclose(fp);
+    return found;
+}
+
+static void write_cache(Device d, TiledShape s)
+{
+    const char* path = getenv("ASTAROTH_AUTOTUNE_CACHE");
+    if (!path)
+        return;
+    FILE* fp = fopen(path, "a");
+    if (!fp)
+        return;
+    const int3 n = d->local_config.int3_params[AC_nlocal];
+    fprintf(fp, "%d %d %d %d %d %d %d %d %d\n", n.x, n.y, n.z, d->kahan_bits,
+            d->local_config.bool_params[AC_lupwd], d->local_config.bool_params[AC_lentropy], s.tx,
+            s.ty, s.zchunk);
+    fclose(fp);
+}
+
+static AcResult autotune(Device d, hipStream_t s)
+{
+    const int3 n = d->local_config.int3_params[AC_nlocal];
+    TiledShape best{0, 0, 0};
+    if (read_cache(d, &best)) {
+        d->shape = best;
+        d->shape_tuned = true;
+        return AC_SUCCESS;
+    }
+    const bool upwd = d->local_config.bool_params[AC_lupwd] != 0;
+    const bool entropy = d->local_config.bool_params[AC_lentropy] != 0;
+    hipEvent_t e0, e1;
+    AC_HIP_CHECK(hipEventCreate(&e0));
+    AC_HIP_CHECK(hipEventCreate(&e1));
+    // Dry runs read `in` and write `out` and `w` at step 1: w is restored from a copy afterwards,
+    // and Kahan compensation is disabled for the trial launches so it stays untouched.
+    const size_t wbytes = d->count * sizeof(AcReal) * NF;
+    void* wsave = nullptr;
+    AC_HIP_CHECK(hipMalloc(&wsave, wbytes));
+    AC_HIP_CHECK(hipMemcpyAsync(wsave, d->slab_w, wbytes, hipMemcpyDeviceToDevice, s));
+    SolveArgs<AcReal> a;
+    fill_solve_args(d, a, 1, (AcReal)1e-9);
+    float best_ms = 1e30f;
+    // direct kernel as the baseline candidate
+    {
+        launch_solve_direct<AcReal>(a, upwd, entropy, 0, s);
+        AC_HIP_CHECK(hipEventRecord(e0, s));
+        for (int r = 0; r < 2; ++r)
+            launch_solve_direct<AcReal>(a, upwd, entropy, 0, s);
+        AC_HIP_CHECK(hipEventRecord(e1, s));
+        AC_HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        AC_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        best_ms = ms;
+    }
+    for (int i = 0; i < tiled_num_shapes(); ++i) {
+        const TiledShape sh = tiled_shape(i);
+        if (!tiled_shape_fits(sh, n.x, n.y, n.z))
+            continue;
+        if (!launch_solve_tiled<AcReal>(a, upwd, entropy, 0, sh, s))
+            continue;
+        AC_HIP_CHECK(hipEventRecord(e0, s));
+        for (int r = 0; r < 2; ++r)
+            launch_solve_tiled<AcReal>(a, upwd, entropy, 0, sh, s);
+        AC_HIP_CHECK(hipEventRecord(e1, s));
+        AC_HIP_CHECK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        AC_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = sh;
+        }
+    }
+    AC_HIP_CHECK(hipGetLastError());
+    AC_HIP_CHECK(hipMemcpyAsync(d->slab_w, wsave, wbytes, hipMemcpyDeviceToDevice, s));
+    AC_HIP_CHECK(hipStreamSynchronize(s));
+    hipFree(wsave);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    d->shape = best;
+    d->shape_tuned = true;
+    write_cache(d, best);
+    return AC_SUCCESS;
+}
+
+static AcResult ensure_shape(Device d, hipStream_t s)
+{
+    if (d->shape_tuned)
+        return AC_SUCCESS;
+    return autotune(d, s);
+}
+
+extern "C" AcResult acDeviceAutotune(const Device d, const Stream stream)
+{
+    AC_HIP_CHECK(hipSetDevice(d->id));
+    d->shape_tuned = false;
+    return autotune(d, get_stream(d, stream));
+}

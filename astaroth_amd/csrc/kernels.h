@@ -1,0 +1,77 @@
+// kernels.h -- launchers of the HIP kernels of the MHD hot path, called by the device layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "mhd_common.h"
+
+#ifndef AC_DOUBLE_PRECISION
+#define AC_DOUBLE_PRECISION 0
+#endif
+
+namespace acamd {
+
+#if AC_DOUBLE_PRECISION
+typedef double AcRealT;
+#else
+typedef float AcRealT;
+#endif
+
+// Wave-level max of a non-negative double, one atomic per wave64 into `dst`.
+// Every lane of the calling wave must reach this call.
+__device__ __forceinline__ void wave_max_to_global(double v, unsigned long long* dst)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        v = fmax(v, __shfl_xor(v, off, 64));
+    const unsigned lane = (threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z)) & 63u;
+    if (lane == 0u)
+        atomic_max_nonneg(dst, v);
+}
+
+// ---- solve ----
+template <typename T>
+void launch_solve_direct(const SolveArgs<T>& a, bool upwd, bool entropy, int kahan_bits,
+                         hipStream_t stream);
+
+// LDS-tiled 2.5D solve. Returns false when the launch shape does not fit the grid.
+struct TiledShape {
+    int tx, ty, zchunk;
+};
+template <typename T>
+bool launch_solve_tiled(const SolveArgs<T>& a, bool upwd, bool entropy, int kahan_bits,
+                        TiledShape shape, hipStream_t stream);
+// Candidate launch shapes of the tiled kernel (for the autotuner).
+int tiled_num_shapes();
+TiledShape tiled_shape(int i);
+bool tiled_shape_fits(TiledShape s, int nx, int ny, int nz);
+
+// ---- periodic fill: every halo cell of every listed field, one launch ----
+template <typename T>
+void launch_periodic_fill(T* const* fields, int nfields, int nx, int ny, int nz, hipStream_t s);
+
+// ---- reductions ----
+// map: AcMapKernel value; op: AcReduceOp value. Interior cells only. Result is the raw
+// FP64 reduction (no RMS post-processing), written to *result_dev (device memory).
+template <typename T>
+void launch_reduce(const T* const* bufs, int map, int op, int nx, int ny, int nz,
+                   double* partials, int max_partials, double* result_dev, hipStream_t s);
+int reduce_num_partials(int nx, int ny, int nz);
+
+// One-pass diagnostics: writes 4 * NF + 4 doubles to out_dev:
+// [min x8][max x8][sum sq x8][sum |u|^2][max |u|^2][sum |B|^2][max |B|^2]
+template <typename T>
+void launch_diagnostics(const T* const* fields, const Coeffs<T>& C, int nx, int ny, int nz,
+                        double* partials, int max_partials, double* out_dev, hipStream_t s);
+
+// max |u|^2 over the interior into umax2 (atomic max, caller zeroes it first).
+template <typename T>
+void launch_umax2(const T* const* uu, int nx, int ny, int nz, unsigned long long* umax2,
+                  hipStream_t s);
+
+// Fill n values at dst with v.
+template <typename T>
+void launch_fill(T* dst, size_t n, T v, hipStream_t s);
+void launch_zero_bytes(void* dst, size_t nbytes, hipStream_t s);
+
+}  // namespace acamd

@@ -1,0 +1,261 @@
+// mhd_common.h -- device-side pieces shared by every solve kernel of the MHD hot path:
+// launch arguments, the per-cell right-hand side, and the fused 2N-RK3 state update.
+//
+// Physics follows ref acc-runtime/samples/mhd_modular/mhdsolver.ac:279-426 (continuity,
+// momentum, induction, entropy with heat conduction) and ref src/utils/modelsolver.cc:772-964.
+// Stencils are the 6th-order ones of ref acc-runtime/stdlib/derivs.h:374-427,591-752.
+// The RK3 coefficients are Williamson's 2N set, ref acc-runtime/stdlib/integrators.h:11-12.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace acamd {
+
+constexpr int NF = 8;  // lnrho, ux, uy, uz, ax, ay, az, ss
+constexpr int R = 3;   // stencil radius
+
+// Scalar parameters of the right-hand side, in the compute type T.
+template <typename T>
+struct PhysParams {
+    T cs2_sound, nu, zeta, eta, mu0, inv_mu0, cp, inv_cp, gamma, gm1, lnT0, lnrho0, K;
+    T gamma_inv_cp;  // gamma / cp
+    T eta_mu0;       // eta * mu0
+    T nu_third;      // nu / 3
+};
+
+// Stencil coefficients already multiplied by the grid spacing factors, per axis (x, y, z).
+// d1[a][o]: first derivative, offset o = 1..3 (antisymmetric), d1[a][0] unused.
+// d2[a][o]: second derivative, o = 0 is the centre weight.
+// up[a][o]: upwind (hyperdiffusion) operator, o = 0 is the centre weight.
+// mx[p][o]: mixed derivative along the diagonals of plane p (0 = xy, 1 = xz, 2 = yz).
+template <typename T>
+struct Coeffs {
+    T d1[3][4];
+    T d2[3][4];
+    T up[3][4];
+    T mx[3][4];
+};
+
+// Everything one fused substep launch needs. Passed by value (well under the 4 KiB limit).
+template <typename T>
+struct SolveArgs {
+    const T* in[NF];   // state f_s, halos filled
+    T* out[NF];        // f_{s+1}, interior written
+    T* w[NF];          // 2N register, updated in place
+    void* comp[NF];    // Kahan compensation (FP32 build, optional), updated in place
+    int nx, ny, nz;    // interior extent
+    int mx, my, mz;    // extent with halos
+    long long sz;      // mx * my
+    Coeffs<T> C;
+    PhysParams<T> P;
+    T dt;              // used when dt_dev == nullptr
+    T alpha, beta;     // alpha_s, beta_s
+    int step;          // substep 0, 1, 2
+    // Device-resident CFL timestep (nullptr = use dt): dt = cdt * dsmin / (cs + sqrt(umax2[0]))
+    const double* umax2_cur;
+    unsigned long long* umax2_next;  // step 2: atomic max of |u_new|^2 (double bits)
+    double* time_acc;                // step 2: accumulated simulated time
+    double cfl_cdt, cfl_dsmin, cfl_cs;
+};
+
+__device__ __forceinline__ float ac_exp(float x) { return expf(x); }
+__device__ __forceinline__ double ac_exp(double x) { return exp(x); }
+__device__ __forceinline__ float ac_abs(float x) { return fabsf(x); }
+__device__ __forceinline__ double ac_abs(double x) { return fabs(x); }
+
+// Derivative quantities of one cell that the right-hand side depends on.
+// gu[i][j] = d_j u_i, ga[i][j] = d_j A_i. upl = |u| . upwind-grad(lnrho),
+// upu[i] = sum_j upwind_j(u_i) |u_j| (ref modelsolver.cc:776-779,843-846).
+template <typename T>
+struct CellDerivs {
+    T lr, glr[3], laplr, upl;
+    T u[3], gu[3][3], lapu[3], gdu[3], upu[3];
+    T ga[3][3], lapa[3], gda[3];
+    T s, gs[3], laps;
+};
+
+// Right-hand side of the eight evolved fields at one cell.
+template <typename T, bool UPWD, bool ENTROPY>
+__device__ __forceinline__ void mhd_rhs(const CellDerivs<T>& d, const PhysParams<T>& P, T rhs[NF])
+{
+    const T divu = d.gu[0][0] + d.gu[1][1] + d.gu[2][2];
+    // continuity (ref mhdsolver.ac:334-340)
+    T cont = -(d.u[0] * d.glr[0] + d.u[1] * d.glr[1] + d.u[2] * d.glr[2]) - divu;
+    if (UPWD)
+        cont += d.upl;
+    rhs[0] = cont;
+
+    // traceless rate of strain (ref operators.h:145-161)
+    const T third = T(1) / T(3);
+    const T S00 = T(2) * third * d.gu[0][0] - third * (d.gu[1][1] + d.gu[2][2]);
+    const T S11 = T(2) * third * d.gu[1][1] - third * (d.gu[0][0] + d.gu[2][2]);
+    const T S22 = T(2) * third * d.gu[2][2] - third * (d.gu[0][0] + d.gu[1][1]);
+    const T S01 = T(0.5) * (d.gu[0][1] + d.gu[1][0]);
+    const T S02 = T(0.5) * (d.gu[0][2] + d.gu[2][0]);
+    const T S12 = T(0.5) * (d.gu[1][2] + d.gu[2][1]);
+
+    // magnetic field and current (ref mhdsolver.ac:354-358)
+    const T B0 = d.ga[2][1] - d.ga[1][2];
+    const T B1 = d.ga[0][2] - d.ga[2][0];
+    const T B2 = d.ga[1][0] - d.ga[0][1];
+    const T j0 = P.inv_mu0 * (d.gda[0] - d.lapa[0]);
+    const T j1 = P.inv_mu0 * (d.gda[1] - d.lapa[1]);
+    const T j2 = P.inv_mu0 * (d.gda[2] - d.lapa[2]);
+
+    const T rho = ac_exp(d.lr);
+    const T inv_rho = T(1) / rho;
+
+    T cs2;
+    T lnT_rel = T(0);  // lnT - lnT0
+    if (ENTROPY) {
+        lnT_rel = P.gamma_inv_cp * d.s + P.gm1 * (d.lr - P.lnrho0);
+        cs2 = P.cs2_sound * ac_exp(lnT_rel);
+    }
+    else {
+        cs2 = P.cs2_sound;
+    }
+
+    const T jxB0 = j1 * B2 - j2 * B1;
+    const T jxB1 = j2 * B0 - j0 * B2;
+    const T jxB2 = j0 * B1 - j1 * B0;
+    const T jxB[3] = {jxB0, jxB1, jxB2};
+    const T Srow[3][3] = {{S00, S01, S02}, {S01, S11, S12}, {S02, S12, S22}};
+
+    // momentum (ref mhdsolver.ac:342-384)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        T m = -(d.gu[i][0] * d.u[0] + d.gu[i][1] * d.u[1] + d.gu[i][2] * d.u[2]);
+        if (UPWD)
+            m += d.upu[i];
+        const T press = ENTROPY ? (P.inv_cp * d.gs[i] + d.glr[i]) : d.glr[i];
+        m -= cs2 * press;
+        m += inv_rho * jxB[i];
+        const T Sg = Srow[i][0] * d.glr[0] + Srow[i][1] * d.glr[1] + Srow[i][2] * d.glr[2];
+        m += P.nu * (d.lapu[i] + third * d.gdu[i] + T(2) * Sg);
+        m += P.zeta * d.gdu[i];
+        rhs[1 + i] = m;
+    }
+
+    // induction (ref mhdsolver.ac:279-281): u x B + eta lap A
+    rhs[4] = (d.u[1] * B2 - d.u[2] * B1) + P.eta * d.lapa[0];
+    rhs[5] = (d.u[2] * B0 - d.u[0] * B2) + P.eta * d.lapa[1];
+    rhs[6] = (d.u[0] * B1 - d.u[1] * B0) + P.eta * d.lapa[2];
+
+    // entropy (ref mhdsolver.ac:387-426)
+    if (ENTROPY) {
+        // 1 / (rho T) = exp(-(lnrho + lnT))
+        const T inv_pT = ac_exp(-(d.lr + lnT_rel + P.lnT0));
+        const T contractS = S00 * S00 + S11 * S11 + S22 * S22 +
+                            T(2) * (S01 * S01 + S02 * S02 + S12 * S12);
+        const T RHS = P.eta_mu0 * (j0 * j0 + j1 * j1 + j2 * j2) + T(2) * rho * P.nu * contractS +
+                      P.zeta * rho * divu * divu;
+        const T first = P.gamma_inv_cp * d.laps + P.gm1 * d.laplr;
+        T st = T(0);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const T second = P.gamma_inv_cp * d.gs[i] + P.gm1 * d.glr[i];
+            const T thirdv = P.gamma * (P.inv_cp * d.gs[i] + d.glr[i]) - d.glr[i];
+            st += second * thirdv;
+        }
+        // cp * chi with chi = K / (rho cp)
+        const T heat = P.K * inv_rho * (first + st);
+        rhs[7] = -(d.u[0] * d.gs[0] + d.u[1] * d.gs[1] + d.u[2] * d.gs[2]) + inv_pT * RHS + heat;
+    }
+    else {
+        rhs[7] = T(0);
+    }
+}
+
+// ---- Kahan compensation storage (FP32 build) --------------------------------------------
+// The compensation c of a value f obeys |c| <= ulp(f)/2. It is stored either as a float
+// (KB = 32) or as a signed integer in units of ulp(f) * 2^-KB (KB = 8 or 16), so one extra
+// byte per value keeps eight more bits of the state.
+__device__ __forceinline__ int ulp_exp_f32(float f)
+{
+    const uint32_t e = (__float_as_uint(f) >> 23) & 0xffu;
+    return (int)(e > 0u ? e : 1u) - 150;  // ulp(f) = 2^ulp_exp
+}
+
+template <int KB>
+__device__ __forceinline__ float comp_load(const void* base, long long c, float f)
+{
+    if (KB == 32) {
+        return ((const float*)base)[c];
+    }
+    else if (KB == 16) {
+        const int q = ((const int16_t*)base)[c];
+        return ldexpf((float)q, ulp_exp_f32(f) - 16);
+    }
+    else {
+        const int q = ((const int8_t*)base)[c];
+        return ldexpf((float)q, ulp_exp_f32(f) - 8);
+    }
+}
+
+template <int KB>
+__device__ __forceinline__ void comp_store(void* base, long long c, float comp, float f)
+{
+    if (KB == 32) {
+        ((float*)base)[c] = comp;
+    }
+    else if (KB == 16) {
+        float q = rintf(ldexpf(comp, 16 - ulp_exp_f32(f)));
+        q = fminf(fmaxf(q, -32767.f), 32767.f);
+        ((int16_t*)base)[c] = (int16_t)q;
+    }
+    else {
+        float q = rintf(ldexpf(comp, 8 - ulp_exp_f32(f)));
+        q = fminf(fmaxf(q, -127.f), 127.f);
+        ((int8_t*)base)[c] = (int8_t)q;
+    }
+}
+
+__device__ __forceinline__ float mul_rn(float a, float b) { return __fmul_rn(a, b); }
+__device__ __forceinline__ double mul_rn(double a, double b) { return __dmul_rn(a, b); }
+__device__ __forceinline__ float add_rn(float a, float b) { return __fadd_rn(a, b); }
+__device__ __forceinline__ double add_rn(double a, double b) { return __dadd_rn(a, b); }
+
+// Fused 2N-RK3 update of one value (ref integrators.h:87-101, modelsolver.cc:1126-1159):
+//   w' = alpha_s w + dt rhs        (w is not read at s = 0: alpha_0 = 0)
+//   f' = f + beta_s w'             (plain: rounded product, then rounded sum, like the
+//                                   reference's final pass; KB > 0: Kahan-compensated)
+// w' is stored for s < 2 only (it is dead after the last substep).
+template <typename T, int KB>
+__device__ __forceinline__ void rk_update(const SolveArgs<T>& a, int q, long long c, T f, T rhs,
+                                          T dt, T* fout)
+{
+    T wn = dt * rhs;
+    if (a.step > 0)
+        wn = a.alpha * a.w[q][c] + wn;
+    if (a.step < 2)
+        a.w[q][c] = wn;
+    if (KB == 0) {
+        *fout = add_rn(f, mul_rn(a.beta, wn));
+    }
+    else {
+        const float cc = comp_load<KB>(a.comp[q], c, f);
+        const float y = a.beta * wn - cc;
+        const float t = f + y;
+        const float cn = (t - f) - y;
+        comp_store<KB>(a.comp[q], c, cn, t);
+        *fout = t;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T solve_dt(const SolveArgs<T>& a)
+{
+    if (a.umax2_cur == nullptr)
+        return a.dt;
+    const double umax = sqrt(*a.umax2_cur);
+    return (T)(a.cfl_cdt * a.cfl_dsmin / (a.cfl_cs + umax));
+}
+
+// Block-wide max of |u|^2 into umax2_next (step 2 of the device-dt path), plus time.
+__device__ __forceinline__ void atomic_max_nonneg(unsigned long long* addr, double v)
+{
+    atomicMax(addr, (unsigned long long)__double_as_longlong(v));
+}
+
+}  // namespace acamd

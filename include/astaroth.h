@@ -1,0 +1,452 @@
+/*
+ * astaroth.h -- C ABI of the MI355X-native single-device MHD hot path.
+ *
+ * Source-compatible subset of Astaroth's public device API for the
+ * 6th-order MHD solve, the 2N-storage RK3 substep, the periodic halo fill and
+ * the device reductions. Every function listed here is exported with C
+ * linkage by libastaroth_fp32.so (AcReal = float) and libastaroth_fp64.so
+ * (AcReal = double); the build picks the precision exactly as the reference's
+ * DOUBLE_PRECISION option does (ref include/acreal.h, CMakeLists.txt:32-62).
+ *
+ * Reference interfaces replaced (file:line in the reference tree):
+ *   types       include/astaroth_base.h:43-86,134-135, include/host_datatypes.h:10,
+ *               acc-runtime/built-in/typedefs.h:218-222, acc-runtime/api/acc_runtime.h:92-100,157-175
+ *   device API  include/astaroth_device.h:13-246 (implemented in src/core/device.cc)
+ *   host API    include/astaroth_helpers.h:291-327, include/astaroth_utils.h:47-83
+ *
+ * Parameters keep the reference's names (AC_nu_visc, AC_ds, AC_nlocal, ...),
+ * stored in fixed enum-indexed arrays. In the reference AcMeshInfo is generated
+ * by the DSL compiler and has no stable binary layout, so compatibility is at
+ * the source level: same function names, argument meaning and error codes.
+ */
+#ifndef ASTAROTH_AMD_H
+#define ASTAROTH_AMD_H
+
+#include <stdbool.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+#define AC_BEGIN_C_DECLARATIONS extern "C" {
+#define AC_END_C_DECLARATIONS }
+#else
+#define AC_BEGIN_C_DECLARATIONS
+#define AC_END_C_DECLARATIONS
+#endif
+
+#ifndef AC_DOUBLE_PRECISION
+#define AC_DOUBLE_PRECISION 0
+#endif
+
+#if AC_DOUBLE_PRECISION
+typedef double AcReal;
+#define AC_REAL_EPSILON (2.2204460492503131e-16)
+#else
+typedef float AcReal;
+#define AC_REAL_EPSILON (1.19209290e-07f)
+#endif
+
+#define STENCIL_ORDER (6)
+#define NGHOST (STENCIL_ORDER / 2)
+
+/* ---- basic types (ref typedefs.h:7-12,218-222; host_datatypes.h:10) ---- */
+#if defined(__HIP__)
+#include <hip/hip_runtime.h> /* HIP's int3 (12 bytes, 4-byte aligned, same layout as below) */
+#elif !defined(AC_HAVE_INT3)
+#define AC_HAVE_INT3
+typedef struct {
+    int x, y, z;
+} int3;
+#endif
+
+typedef struct {
+    AcReal x, y, z;
+} AcReal3;
+
+typedef struct {
+    size_t x, y, z;
+} Volume;
+
+typedef enum { AC_SUCCESS = 0, AC_FAILURE = 1, AC_NOT_ALLOCATED = 2 } AcResult;
+
+/* ---- fields (ref mhdsolver.ac:125,136,140: declaration order) ---- */
+typedef enum {
+    VTXBUF_LNRHO = 0,
+    VTXBUF_UUX,
+    VTXBUF_UUY,
+    VTXBUF_UUZ,
+    VTXBUF_AX,
+    VTXBUF_AY,
+    VTXBUF_AZ,
+    VTXBUF_ENTROPY,
+    NUM_VTXBUF_HANDLES
+} VertexBufferHandle;
+typedef VertexBufferHandle Field;
+#define NUM_FIELDS (NUM_VTXBUF_HANDLES)
+#define NUM_ALL_FIELDS (NUM_VTXBUF_HANDLES)
+
+/* ---- parameters (ref built-in/variables.h, mhdsolver.ac:196-222, config/astaroth.conf) ---- */
+typedef enum {
+    AC_max_steps = 0,
+    AC_save_steps,
+    AC_start_step,
+    AC_kahan_bits,       /* FP32 build: bits of stored Kahan compensation per value (0 = off, 8, 16, 32) */
+    AC_solve_tile_x,     /* launch shape of the solve kernel (0 = autotune) */
+    AC_solve_tile_y,
+    AC_solve_zchunk,
+    NUM_INT_PARAMS
+} AcIntParam;
+
+typedef enum {
+    AC_ngrid = 0,
+    AC_nlocal,
+    AC_mlocal,
+    AC_mgrid,
+    AC_nmin,
+    AC_nlocal_max,
+    AC_ngrid_max,
+    AC_multigpu_offset,
+    NUM_INT3_PARAMS
+} AcInt3Param;
+
+typedef enum {
+    AC_cdt = 0,
+    AC_cdtv,
+    AC_cdts,
+    AC_nu_visc,
+    AC_cs_sound,
+    AC_zeta,
+    AC_eta,
+    AC_mu0,
+    AC_chi,
+    AC_cp_sound,
+    AC_gamma,
+    AC_lnT0,
+    AC_lnrho0,
+    AC_K_heatcond,
+    AC_cs2_sound,
+    AC_dsmin,
+    AC_current_time,
+    AC_dt,
+    NUM_REAL_PARAMS
+} AcRealParam;
+
+typedef enum {
+    AC_ds = 0,
+    AC_inv_ds,
+    AC_inv_ds_2,
+    AC_len,
+    NUM_REAL3_PARAMS
+} AcReal3Param;
+
+typedef enum {
+    AC_lupwd = 0,        /* ref hostdefine LUPWD (mhdsolver.ac:101): 6th-order upwind hyperdiffusion */
+    AC_lentropy,         /* ref hostdefine LENTROPY (mhdsolver.ac:99): evolve VTXBUF_ENTROPY */
+    AC_additive_timestep,
+    AC_periodic_grid_all,
+    NUM_BOOL_PARAMS
+} AcBoolParam;
+
+typedef struct AcMeshInfo {
+    int int_params[NUM_INT_PARAMS];
+    int3 int3_params[NUM_INT3_PARAMS];
+    AcReal real_params[NUM_REAL_PARAMS];
+    AcReal3 real3_params[NUM_REAL3_PARAMS];
+    int bool_params[NUM_BOOL_PARAMS];
+#ifdef __cplusplus
+    int& operator[](const AcIntParam p) { return int_params[p]; }
+    const int& operator[](const AcIntParam p) const { return int_params[p]; }
+    int3& operator[](const AcInt3Param p) { return int3_params[p]; }
+    const int3& operator[](const AcInt3Param p) const { return int3_params[p]; }
+    AcReal& operator[](const AcRealParam p) { return real_params[p]; }
+    const AcReal& operator[](const AcRealParam p) const { return real_params[p]; }
+    AcReal3& operator[](const AcReal3Param p) { return real3_params[p]; }
+    const AcReal3& operator[](const AcReal3Param p) const { return real3_params[p]; }
+    int& operator[](const AcBoolParam p) { return bool_params[p]; }
+    const int& operator[](const AcBoolParam p) const { return bool_params[p]; }
+#endif
+} AcMeshInfo;
+
+/* ---- host mesh (ref astaroth_base.h:43-47) ---- */
+typedef struct {
+    AcReal* vertex_buffer[NUM_ALL_FIELDS];
+    AcMeshInfo info;
+} AcMesh;
+
+/* ---- streams (ref astaroth_base.h:49-86). All are created at default priority. ---- */
+typedef enum {
+    STREAM_0, STREAM_1, STREAM_2, STREAM_3, STREAM_4, STREAM_5, STREAM_6, STREAM_7,
+    STREAM_8, STREAM_9, STREAM_10, STREAM_11, STREAM_12, STREAM_13, STREAM_14, STREAM_15,
+    STREAM_16, STREAM_17, STREAM_18, STREAM_19, STREAM_20, STREAM_21, STREAM_22, STREAM_23,
+    STREAM_24, STREAM_25, STREAM_26, STREAM_27, STREAM_28, STREAM_29, STREAM_30, STREAM_31,
+    STREAM_ALL
+} Stream;
+#define STREAM_DEFAULT (STREAM_0)
+#define NUM_STREAMS (32)
+
+typedef struct device_s* Device;
+
+/* ---- reductions (ref acc_runtime.h:92-100, stdlib/reduction.h) ---- */
+typedef enum {
+    AC_REDUCE_OP_NO_REDUCE = 0,
+    AC_REDUCE_OP_MIN,
+    AC_REDUCE_OP_MAX,
+    AC_REDUCE_OP_SUM
+} AcReduceOp;
+
+typedef enum { AC_NO_REDUCE_POST_PROCESSING = 0, AC_RMS = 1 } AcReducePostProcessingOp;
+
+typedef enum {
+    AC_NULL_KERNEL = 0,
+    AC_MAP_VTXBUF,
+    AC_MAP_VTXBUF_SQUARE,
+    AC_MAP_VTXBUF_EXP_SQUARE,
+    AC_MAP_VTXBUF3_NORM,
+    AC_MAP_VTXBUF3_SQUARE,
+    AC_MAP_VTXBUF3_EXP_SQUARE,
+    AC_MAP_VTXBUF4_ALFVEN_NORM,
+    AC_MAP_VTXBUF4_ALFVEN_SQUARE
+} AcMapKernel;
+
+typedef struct {
+    AcReduceOp reduce_op;
+    AcReducePostProcessingOp post_processing_op;
+    AcMapKernel map_vtxbuf_single;
+    AcMapKernel map_vtxbuf_vec;
+    AcMapKernel map_vtxbuf_vec_scal;
+    const char* name;
+} AcReduction;
+
+static const AcReduction RTYPE_MAX = {AC_REDUCE_OP_MAX, AC_NO_REDUCE_POST_PROCESSING, AC_MAP_VTXBUF,
+                                      AC_MAP_VTXBUF3_NORM, AC_NULL_KERNEL, "RTYPE_MAX"};
+static const AcReduction RTYPE_MIN = {AC_REDUCE_OP_MIN, AC_NO_REDUCE_POST_PROCESSING, AC_MAP_VTXBUF,
+                                      AC_MAP_VTXBUF3_NORM, AC_NULL_KERNEL, "RTYPE_MIN"};
+static const AcReduction RTYPE_SUM = {AC_REDUCE_OP_SUM, AC_NO_REDUCE_POST_PROCESSING, AC_MAP_VTXBUF,
+                                      AC_MAP_VTXBUF3_NORM, AC_NULL_KERNEL, "RTYPE_SUM"};
+static const AcReduction RTYPE_RMS = {AC_REDUCE_OP_SUM, AC_RMS, AC_MAP_VTXBUF_SQUARE,
+                                      AC_MAP_VTXBUF3_SQUARE, AC_NULL_KERNEL, "RTYPE_RMS"};
+static const AcReduction RTYPE_RMS_EXP = {AC_REDUCE_OP_SUM, AC_RMS, AC_MAP_VTXBUF_EXP_SQUARE,
+                                          AC_MAP_VTXBUF3_EXP_SQUARE, AC_NULL_KERNEL,
+                                          "RTYPE_RMS_EXP"};
+static const AcReduction RTYPE_ALFVEN_MAX = {AC_REDUCE_OP_MAX, AC_NO_REDUCE_POST_PROCESSING,
+                                             AC_NULL_KERNEL, AC_NULL_KERNEL,
+                                             AC_MAP_VTXBUF4_ALFVEN_NORM, "RTYPE_ALFVEN_MAX"};
+static const AcReduction RTYPE_ALFVEN_MIN = {AC_REDUCE_OP_MIN, AC_NO_REDUCE_POST_PROCESSING,
+                                             AC_NULL_KERNEL, AC_NULL_KERNEL,
+                                             AC_MAP_VTXBUF4_ALFVEN_NORM, "RTYPE_ALFVEN_MIN"};
+static const AcReduction RTYPE_ALFVEN_RMS = {AC_REDUCE_OP_SUM, AC_RMS, AC_NULL_KERNEL,
+                                             AC_NULL_KERNEL, AC_MAP_VTXBUF4_ALFVEN_SQUARE,
+                                             "RTYPE_ALFVEN_RMS"};
+
+/* One-pass diagnostics of every field (no reference counterpart as one call; the same
+ * quantities are what ref samples/standalone_mpi/main.cc:256-391 prints per step). */
+typedef struct {
+    double min[NUM_VTXBUF_HANDLES];
+    double max[NUM_VTXBUF_HANDLES];
+    double rms[NUM_VTXBUF_HANDLES];
+    double uu_rms;   /* sqrt(<|u|^2>) */
+    double uu_max;   /* max |u| */
+    double bb_rms;   /* sqrt(<|curl A|^2>) */
+    double bb_max;   /* max |curl A| */
+} AcDiagnostics;
+
+AC_BEGIN_C_DECLARATIONS
+
+/* ===================== Device interface (ref include/astaroth_device.h) ===================== */
+
+/* ref astaroth_device.h:13, device.cc:508-593 */
+AcResult acDeviceCreate(const int id, const AcMeshInfo device_config, Device* device);
+/* ref astaroth_device.h:15, device.cc:601-632 */
+AcResult acDeviceDestroy(Device* device);
+/* ref astaroth_device.h:18 */
+AcResult acDeviceResetMesh(const Device device, const Stream stream);
+/* ref astaroth_device.h:20 */
+AcResult acDevicePrintInfo(const Device device);
+/* ref astaroth_device.h:24 */
+AcResult acDeviceSynchronizeStream(const Device device, const Stream stream);
+/* ref astaroth_device.h:26, device.cc:634-651 */
+AcResult acDeviceSwapBuffer(const Device device, const VertexBufferHandle handle);
+/* ref astaroth_device.h:28, device.cc:653-663 */
+AcResult acDeviceSwapBuffers(const Device device);
+/* ref astaroth_device.h:32 */
+AcResult acDeviceLoadScalarUniform(const Device device, const Stream stream,
+                                   const AcRealParam param, const AcReal value);
+/* ref astaroth_device.h:47 */
+AcResult acDeviceLoadVectorUniform(const Device device, const Stream stream,
+                                   const AcReal3Param param, const AcReal3 value);
+/* ref astaroth_device.h:50 */
+AcResult acDeviceLoadIntUniform(const Device device, const Stream stream, const AcIntParam param,
+                                const int value);
+/* ref astaroth_device.h:52 */
+AcResult acDeviceLoadBoolUniform(const Device device, const Stream stream,
+                                 const AcBoolParam param, const bool value);
+/* ref astaroth_device.h:55 */
+AcResult acDeviceLoadInt3Uniform(const Device device, const Stream stream,
+                                 const AcInt3Param param, const int3 value);
+/* ref astaroth_device.h:58 */
+AcResult acDeviceStoreScalarUniform(const Device device, const Stream stream,
+                                    const AcRealParam param, AcReal* value);
+/* ref astaroth_device.h:61 */
+AcResult acDeviceStoreVectorUniform(const Device device, const Stream stream,
+                                    const AcReal3Param param, AcReal3* value);
+/* ref astaroth_device.h:64, device.cc:394-413 */
+AcResult acDeviceLoadMeshInfo(const Device device, const AcMeshInfo device_config);
+/* ref astaroth_device.h:67, device.cc:665-686 */
+AcResult acDeviceLoadVertexBufferWithOffset(const Device device, const Stream stream,
+                                            const AcMesh host_mesh,
+                                            const VertexBufferHandle vtxbuf_handle, const int3 src,
+                                            const int3 dst, const int num_vertices);
+/* ref astaroth_device.h:73 */
+AcResult acDeviceLoadMeshWithOffset(const Device device, const Stream stream,
+                                    const AcMesh host_mesh, const int3 src, const int3 dst,
+                                    const int num_vertices);
+/* ref astaroth_device.h:77, device.cc:702-720 */
+AcResult acDeviceLoadVertexBuffer(const Device device, const Stream stream, const AcMesh host_mesh,
+                                  const VertexBufferHandle vtxbuf_handle);
+/* ref astaroth_device.h:80, device.cc:736-745 */
+AcResult acDeviceLoadMesh(const Device device, const Stream stream, const AcMesh host_mesh);
+/* ref astaroth_device.h:93 */
+AcResult acDeviceSetVertexBuffer(const Device device, const Stream stream,
+                                 const VertexBufferHandle handle, const AcReal value);
+/* ref astaroth_device.h:96 */
+AcResult acDeviceFlushOutputBuffers(const Device device, const Stream stream);
+/* ref astaroth_device.h:98, device.cc:788-810 */
+AcResult acDeviceStoreVertexBufferWithOffset(const Device device, const Stream stream,
+                                             const VertexBufferHandle vtxbuf_handle,
+                                             const int3 src, const int3 dst,
+                                             const int num_vertices, AcMesh* host_mesh);
+/* ref astaroth_device.h:102 */
+AcMeshInfo acDeviceGetConfig(const Device device);
+/* ref astaroth_device.h:246 */
+AcMeshInfo acDeviceGetLocalConfig(const Device device);
+/* ref astaroth_device.h:108 */
+AcResult acDeviceStoreMeshWithOffset(const Device device, const Stream stream, const int3 src,
+                                     const int3 dst, const int num_vertices, AcMesh* host_mesh);
+/* ref astaroth_device.h:112, device.cc:826-843 */
+AcResult acDeviceStoreVertexBuffer(const Device device, const Stream stream,
+                                   const VertexBufferHandle vtxbuf_handle, AcMesh* host_mesh);
+/* ref astaroth_device.h:115, device.cc:845-854 */
+AcResult acDeviceStoreMesh(const Device device, const Stream stream, AcMesh* host_mesh);
+/* ref astaroth_device.h:133, device.cc:940-996.
+ * One fused launch: w <- alpha_s*w + dt*RHS(in), out <- in + beta_s*w (interior only).
+ * After the call the new state is in `out`; the caller swaps, as with the reference. */
+AcResult acDeviceIntegrateSubstep(const Device device, const Stream stream, const int step_number,
+                                  const Volume start, const Volume end, const AcReal dt);
+/* ref astaroth_device.h:135, device.cc:998-1007 */
+AcResult acDevicePeriodicBoundcondStep(const Device device, const Stream stream,
+                                       const VertexBufferHandle vtxbuf_handle, const Volume start,
+                                       const Volume end);
+/* ref astaroth_device.h:139, device.cc:1009-1017 (here: one launch for all fields) */
+AcResult acDevicePeriodicBoundconds(const Device device, const Stream stream, const Volume start,
+                                    const Volume end);
+/* ref astaroth_device.h:149 */
+AcResult acDeviceReduceScalNoPostProcessing(const Device device, const Stream stream,
+                                            const AcReduction reduction,
+                                            const VertexBufferHandle vtxbuf_handle,
+                                            AcReal* result);
+/* ref astaroth_device.h:153, device.cc:1403-1413 */
+AcResult acDeviceReduceScal(const Device device, const Stream stream, const AcReduction reduction,
+                            const VertexBufferHandle vtxbuf_handle, AcReal* result);
+/* ref astaroth_device.h:157 */
+AcResult acDeviceReduceVecNoPostProcessing(const Device device, const Stream stream,
+                                           const AcReduction reduction,
+                                           const VertexBufferHandle vtxbuf0,
+                                           const VertexBufferHandle vtxbuf1,
+                                           const VertexBufferHandle vtxbuf2, AcReal* result);
+/* ref astaroth_device.h:161, device.cc:1415-1428 */
+AcResult acDeviceReduceVec(const Device device, const Stream stream, const AcReduction reduction,
+                           const VertexBufferHandle vtxbuf0, const VertexBufferHandle vtxbuf1,
+                           const VertexBufferHandle vtxbuf2, AcReal* result);
+/* ref astaroth_device.h:166 */
+AcResult acDeviceReduceVecScalNoPostProcessing(const Device device, const Stream stream,
+                                               const AcReduction reduction,
+                                               const VertexBufferHandle vtxbuf0,
+                                               const VertexBufferHandle vtxbuf1,
+                                               const VertexBufferHandle vtxbuf2,
+                                               const VertexBufferHandle vtxbuf3, AcReal* result);
+/* ref astaroth_device.h:172, device.cc:1430-1443 */
+AcResult acDeviceReduceVecScal(const Device device, const Stream stream,
+                               const AcReduction reduction, const VertexBufferHandle vtxbuf0,
+                               const VertexBufferHandle vtxbuf1, const VertexBufferHandle vtxbuf2,
+                               const VertexBufferHandle vtxbuf3, AcReal* result);
+
+/* ---- Extensions with no one-call counterpart in the reference ---- */
+
+/* Device-resident CFL timestep: dt = courant * dsmin / (cs_sound + max|u|), with max|u|
+ * reduced on the device in FP64. Replaces the host round trip of
+ * ref samples/standalone_mpi/main.cc:395-440 (which reads UU_MAX_ADVEC back each step). */
+AcResult acDeviceUpdateTimestepCFL(const Device device, const Stream stream, const AcReal courant);
+/* Returns the current device-resident dt (synchronises the stream). */
+AcResult acDeviceGetTimestep(const Device device, const Stream stream, double* dt);
+/* As acDeviceIntegrateSubstep, with dt read on the device from the value the last
+ * acDeviceUpdateTimestepCFL (or step_number == 2 of this call) produced. At step_number == 2
+ * the kernel also reduces max|u| of the new state and sets the next dt (courant as given). */
+AcResult acDeviceIntegrateSubstepDeviceDt(const Device device, const Stream stream,
+                                          const int step_number, const AcReal courant);
+/* nsteps full RK3 steps (periodic fill + fused substep + swap, three times per step) with
+ * the device-resident CFL dt; no host synchronisation inside. Accumulated simulated time is
+ * added to *time_out when it is not NULL (synchronises at the end). */
+AcResult acDeviceIntegrateStepsCFL(const Device device, const Stream stream, const int nsteps,
+                                   const AcReal courant, double* time_out);
+/* One pass over the interior: per-field min/max/rms, |u| and |curl A| rms/max (FP64). */
+AcResult acDeviceReduceDiagnostics(const Device device, const Stream stream, AcDiagnostics* out);
+/* Runs the launch autotuner of the solve kernel for the current grid now (it also runs on the
+ * first substep). Writes the chosen shape into the cache file named by
+ * ASTAROTH_AUTOTUNE_CACHE (default: no file). */
+AcResult acDeviceAutotune(const Device device, const Stream stream);
+/* Reports the solve-kernel launch shape in use (tile_x, tile_y, zchunk). */
+AcResult acDeviceGetSolveLaunchShape(const Device device, int* tile_x, int* tile_y, int* zchunk);
+/* Raw device pointers of one field: in (current state), out (next state), w (RK register). */
+AcResult acDeviceGetVertexBufferPtrs(const Device device, const VertexBufferHandle vtxbuf,
+                                     AcReal** in, AcReal** out, AcReal** w);
+/* Solve-kernel timing with HIP events recorded around every solve launch on the stream it is
+ * launched on. enable != 0 starts a new measurement window (up to 4096 launches). */
+AcResult acDeviceSetSolveTiming(const Device device, const int enable);
+/* Synchronises the device and reports the summed device time (ms) and number of solve
+ * launches recorded since the window started. */
+AcResult acDeviceGetSolveTiming(const Device device, double* total_ms, int* launches);
+/* The hipStream_t behind a Stream (for interop with other HIP code; default priority). */
+AcResult acDeviceGetStreamHandle(const Device device, const Stream stream, void** hip_stream);
+
+/* ===================== Host interface (ref include/astaroth_helpers.h, astaroth_utils.h) ===== */
+
+/* ref astaroth_helpers.h:291, src/core/update_params/update_params.cc:19-142 */
+AcResult acHostUpdateParams(AcMeshInfo* config);
+/* ref astaroth_helpers.h:322 */
+AcResult acSetGridMeshDims(const size_t nx, const size_t ny, const size_t nz, AcMeshInfo* info);
+/* ref astaroth_helpers.h:327 */
+AcResult acSetLocalMeshDims(const size_t nx, const size_t ny, const size_t nz, AcMeshInfo* info);
+/* ref astaroth_utils.h:47-83, src/utils/config_loader.cc:198-315: "AC_name = value" lines */
+AcResult acLoadConfig(const char* config_path, AcMeshInfo* config);
+/* Fills a config with the defaults of ref config/astaroth.conf (no file needed). */
+AcResult acSetDefaultConfig(AcMeshInfo* config);
+/* ref astaroth_helpers.h:305, src/core/astaroth.cc:295-303 */
+AcResult acHostMeshCreate(const AcMeshInfo mesh_info, AcMesh* mesh);
+/* ref astaroth_helpers.h:319 */
+AcResult acHostMeshDestroy(AcMesh* mesh);
+/* ref astaroth_helpers.h:307 */
+AcResult acHostMeshCopyVertexBuffers(const AcMesh src, AcMesh dst);
+/* ref astaroth_helpers.h:309 */
+AcResult acHostMeshCopy(const AcMesh src, AcMesh* dst);
+/* ref astaroth_helpers.h:315, src/core/astaroth.cc:382-395: rand()/RAND_MAX per value */
+AcResult acHostMeshRandomize(AcMesh* mesh);
+/* ref src/utils/memory.cc:75-149 */
+AcResult acHostMeshApplyPeriodicBounds(AcMesh* mesh);
+/* Seeded solenoidal Fourier-mode initial condition (no reference counterpart; the reference's
+ * init types are ref astaroth_base.h:88-99). lnrho = ss = 0 (+ uniform perturbation of
+ * amplitude lnrho_perturb in [-a, a) when nonzero); uu and aa each a sum of nmodes modes
+ * ampl * e * cos(k.x + phi), integer k with 1 <= |k| <= kmax, e a unit vector normal to k.
+ * Uses SplitMix64 (seed, seed+1, seed+2 for uu, aa, lnrho). FP64 arithmetic, cast to AcReal. */
+AcResult acHostMeshInitFourier(AcMesh* mesh, const unsigned long long seed, const int nmodes,
+                               const int kmax, const double ampl, const double lnrho_perturb);
+
+/* Library identification: sizeof(AcReal), sizeof(AcMeshInfo), sizeof(AcMesh) of this build and
+ * the offload target it was built for (a caller checks its own struct sizes against these, as
+ * ref src/core/astaroth.cc acVerifyCompatibility does). */
+int acGetRealSize(void);
+size_t acGetMeshInfoSize(void);
+size_t acGetMeshSize(void);
+const char* acGetBuildInfo(void);
+
+AC_END_C_DECLARATIONS
+
+#endif /* ASTAROTH_AMD_H */
