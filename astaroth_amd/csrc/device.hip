@@ -282,12 +282,13 @@ AcResult acDeviceCreate(const int id, const AcMeshInfo device_config, Device* de
     launch_zero_bytes(d->time_acc, sizeof(double), s0);
     AC_HIP_CHECK(hipHostMalloc((void**)&d->host_pinned, sizeof(double) * (2 + kDiagSlots)));
 
+    // AC_solve_tile_x > 0 pins a tiled shape, < 0 pins the direct kernel, 0 autotunes
     d->shape.tx = d->local_config.int_params[AC_solve_tile_x];
     d->shape.ty = d->local_config.int_params[AC_solve_tile_y];
     d->shape.zchunk = d->local_config.int_params[AC_solve_zchunk];
-    if (d->shape.tx > 0 && !tiled_shape_fits(d->shape, n.x, n.y, n.z))
-        d->shape.tx = 0;
-    d->shape_tuned = d->shape.tx > 0;
+    d->shape_tuned = d->shape.tx != 0;
+    if (d->shape.tx < 0 || (d->shape.tx > 0 && !tiled_shape_fits(d->shape, n.x, n.y, n.z)))
+        d->shape = TiledShape{0, 0, 0};
     AC_HIP_CHECK(hipGetLastError());
     AC_HIP_CHECK(hipStreamSynchronize(s0));
     *device_handle = d;
@@ -411,9 +412,9 @@ AcResult acDeviceLoadIntUniform(const Device d, const Stream, const AcIntParam p
             d->shape = s;
             d->shape_tuned = true;
         }
-        else if (s.tx == 0) {
+        else if (s.tx <= 0) {
             d->shape = TiledShape{0, 0, 0};
-            d->shape_tuned = false;
+            d->shape_tuned = s.tx < 0;
         }
         else {
             return fail("acDeviceLoadIntUniform: solve launch shape does not fit the grid");

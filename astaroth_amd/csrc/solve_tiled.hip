@@ -1,17 +1,425 @@
-// solve_tiled.hip -- LDS-tiled 2.5D solve kernel (placeholder until the tiled kernel lands:
-// no candidate shapes, so the device layer uses the direct kernel).
+// solve_tiled.hip -- LDS-tiled, register-rolled (2.5D) fused RK3 substep for gfx950.
+//
+// Same semantics as solve_direct.hip (ref acc-runtime/samples/mhd_modular/mhdsolver.ac:586-636,
+// one fused pass), organised for HBM reuse:
+//
+//  * A workgroup owns a TX x TY column block of the interior and a chunk of ZC planes. It
+//    marches upwards in z; each step stages one xy plane of all eight fields, with its radius-3
+//    halo, in LDS ((TX+6) x (TY+6) values per field), loaded with wave64-coalesced row reads and
+//    software-prefetched one plane ahead through registers.
+//  * Every thread owns one (x, y) column. From the plane in LDS it takes its in-plane stencils
+//    (x and y first, second, upwind and xy-mixed derivatives). Everything that needs other
+//    planes stays in registers:
+//      - a 7-deep queue of its own column's values (z first/second/upwind derivatives);
+//      - "chains" for the xz / yz mixed derivatives, written as
+//            d_xz f(z) = sum_i c_i [ Dx_i f(z+i) - Dx_i f(z-i) ],  Dx_i f = f(x+i) - f(x-i),
+//        where each arriving plane p pushes +-c_i Dx_i f(p) into the accumulators of planes
+//        p -+ i. Only the combinations grad(div u) and grad(div A) are accumulated, with their
+//        in-plane parts (d_xx u_x + d_xy u_y, ...) added when the plane itself arrives;
+//      - a 3-plane delay line of the in-plane derivatives, because the output plane z lags the
+//        newest plane p = z + 3.
+//  * No MFMA: a stencil is not a dense contraction. The kernel is bound by HBM traffic and VALU.
+//  * Workgroups are numbered so that consecutive tiles (which share halo rows) land on the same
+//    XCD and its L2: hardware dispatches workgroup b to XCD b % 8.
 #include "kernels.h"
 
 namespace acamd {
 
-int tiled_num_shapes() { return 0; }
-TiledShape tiled_shape(int) { return TiledShape{0, 0, 0}; }
-bool tiled_shape_fits(TiledShape, int, int, int) { return false; }
+// In-plane quantities of one plane that the output needs three planes later.
+template <typename T>
+struct PlaneVals {
+    T lr_dx, lr_dy, lr_lap, lr_up;       // lnrho: d_x, d_y, d_xx + d_yy, |ux| up_x + |uy| up_y
+    T u_dx[3], u_dy[3], u_lap[3], u_up[3];
+    T ax_dy, ay_dx, az_dx, az_dy, a_lap[3];
+    T s_dx, s_dy, s_lap;
+};
+
+// Chains: 0..2 grad(div u) x, y, z ; 3..5 grad(div A) x, y, z.
+constexpr int NCH = 6;
+
+template <typename T, int LX>
+struct Lds {
+    const T* p;  // this thread's centre in the plane of one field
+    __device__ __forceinline__ T at(int dx, int dy) const { return p[dy * LX + dx]; }
+};
+
+typedef __attribute__((address_space(3))) void* lds_ptr_t;
+
+template <typename T, bool UPWD, bool ENTROPY, int KB, int TX, int TY>
+__global__ void __launch_bounds__(TX* TY) solve_tiled_kernel(SolveArgs<T> a, int zchunk, int ntx,
+                                                             int nty, int nchunks)
+{
+    constexpr int LX = TX + 2 * R;
+    constexpr int LY = TY + 2 * R;
+    constexpr int LP = LX * LY;                  // one field-plane with halo
+    constexpr int NT = TX * TY;
+    constexpr int NW = NT / 64;                  // waves per workgroup
+    constexpr int WPE = (int)sizeof(T) / 4;      // dwords per element
+    constexpr int LPW = LP * WPE;                // dwords per field-plane
+    constexpr int NCHUNK = (LPW + 63) / 64;      // wave-wide DMA chunks (one dword per lane)
+    constexpr int NR = (NCHUNK + NW - 1) / NW;   // chunks per wave per field-plane
+    static_assert(NT % 64 == 0, "workgroup must be whole waves");
+    __shared__ T lds[2][NF * LP];
+
+    // ---- XCD-aware tile numbering: workgroup b runs on XCD b % 8; give each XCD a contiguous
+    // range of tiles (x fastest, then y, then z-chunk) so halo rows are shared within one L2.
+    const int nblocks = ntx * nty * nchunks;
+    const int bid = blockIdx.x;
+    int logical = bid;
+    if ((nblocks & 7) == 0)
+        logical = (bid & 7) * (nblocks >> 3) + (bid >> 3);
+    const int tile_x = logical % ntx;
+    const int rest = logical / ntx;
+    const int tile_y = rest % nty;
+    const int chunk = rest / nty;
+
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int tid = tx + TX * ty;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int x0 = tile_x * TX, y0 = tile_y * TY, z0 = chunk * zchunk;
+    const long long sy = a.mx, sz = a.sz;
+
+    // ---- DMA slots of this lane: chunk j = wave + r * NW covers the dwords [64 j, 64 j + 64) of
+    // a field-plane; lane l moves dword d = 64 j + l, i.e. word d % WPE of element d / WPE.
+    int g_off[NR];  // byte offset of that dword within the array plane
+    bool g_ok[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        const int j = wave + r * NW;
+        const int d = j * 64 + lane;
+        g_ok[r] = (j < NCHUNK) && (d < LPW);
+        const int e = d / WPE, wd = d - e * WPE;
+        const int ly = e / LX, lx = e - ly * LX;
+        g_off[r] = ((x0 + lx) + (int)sy * (y0 + ly)) * (int)sizeof(T) + 4 * wd;
+    }
+    // Issue the DMA of interior plane p (array z index p + R) into buffer b.
+    auto dma_plane = [&](int p, int b) {
+        const long long base = sz * (long long)(p + R);
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            const char* src = (const char*)(a.in[f] + base);
+            char* dst = (char*)&lds[b][f * LP];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                const int j = wave + r * NW;
+                if (j < NCHUNK && g_ok[r])  // j is wave-uniform; g_ok masks the tail lanes
+                    __builtin_amdgcn_global_load_lds((const void*)(src + g_off[r]),
+                                                     (lds_ptr_t)(dst + 256 * j), 4, 0, 0);
+            }
+        }
+    };
+
+    const Coeffs<T>& C = a.C;
+    T q[NF][7];
+    T ch[NCH][7];
+    PlaneVals<T> dly[4];
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+        for (int i = 0; i < 7; ++i)
+            q[f][i] = T(0);
+#pragma unroll
+    for (int c = 0; c < NCH; ++c)
+#pragma unroll
+        for (int i = 0; i < 7; ++i)
+            ch[c][i] = T(0);
+
+    const T dt = solve_dt(a);
+    double u2max = 0.0;
+    const int gx = x0 + tx, gy = y0 + ty;
+    const int ctr = (ty + R) * LX + (tx + R);
+
+    dma_plane(z0 - R, 0);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    const int niter = zchunk + 2 * R;
+    for (int it = 0; it < niter; ++it) {
+        const int p = z0 - R + it;
+        const int cur = it & 1;
+        if (it + 1 < niter)
+            dma_plane(p + 1, cur ^ 1);
+
+        Lds<T, LX> L[NF];
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            L[f].p = &lds[cur][f * LP + ctr];
+            q[f][6] = L[f].at(0, 0);
+        }
+        PlaneVals<T>& nv = dly[3];
+
+        // ---- (1) z-chains: plane p adds +c_i D(p) to plane p - i and -c_i D(p) to plane p + i
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+            const int fx = 1 + 3 * v, fy = 2 + 3 * v, fz = 3 + 3 * v;
+            T dxz = T(0), dyz = T(0), dxx_ = T(0), dyy_ = T(0);
+#pragma unroll
+            for (int i = 1; i <= 3; ++i) {
+                const T Dxz = L[fz].at(i, 0) - L[fz].at(-i, 0);
+                const T Dyz = L[fz].at(0, i) - L[fz].at(0, -i);
+                const T Dxx = L[fx].at(i, 0) - L[fx].at(-i, 0);
+                const T Dyy = L[fy].at(0, i) - L[fy].at(0, -i);
+                const T kx = C.mx[1][i], ky = C.mx[2][i];
+                const T ex = kx * Dxz;              // d_xz v_z -> (grad div v)_x
+                const T ey = ky * Dyz;              // d_yz v_z -> (grad div v)_y
+                const T ez = kx * Dxx + ky * Dyy;   // d_xz v_x + d_yz v_y -> (grad div v)_z
+                ch[3 * v + 0][3 - i] += ex;
+                ch[3 * v + 0][3 + i] -= ex;
+                ch[3 * v + 1][3 - i] += ey;
+                ch[3 * v + 1][3 + i] -= ey;
+                ch[3 * v + 2][3 - i] += ez;
+                ch[3 * v + 2][3 + i] -= ez;
+                dxz += C.d1[0][i] * Dxz;
+                dyz += C.d1[1][i] * Dyz;
+                dxx_ += C.d1[0][i] * Dxx;
+                dyy_ += C.d1[1][i] * Dyy;
+            }
+            if (v == 0) {
+                nv.u_dx[2] = dxz;
+                nv.u_dy[2] = dyz;
+                nv.u_dx[0] = dxx_;
+                nv.u_dy[1] = dyy_;
+            }
+            else {
+                nv.az_dx = dxz;
+                nv.az_dy = dyz;
+                (void)dxx_;
+                (void)dyy_;
+            }
+        }
+
+        // ---- (2) output plane z = p - 3
+        const int z = p - R;
+        if (it >= 2 * R) {
+            const PlaneVals<T>& o = dly[0];
+            T dz[NF], dzz[NF], upz[4];
+#pragma unroll
+            for (int f = 0; f < NF; ++f) {
+                T D[4], S[4];
+#pragma unroll
+                for (int i = 1; i <= 3; ++i) {
+                    D[i] = q[f][3 + i] - q[f][3 - i];
+                    S[i] = q[f][3 + i] + q[f][3 - i];
+                }
+                dz[f] = C.d1[2][1] * D[1] + C.d1[2][2] * D[2] + C.d1[2][3] * D[3];
+                dzz[f] = C.d2[2][0] * q[f][3] + C.d2[2][1] * S[1] + C.d2[2][2] * S[2] + C.d2[2][3] * S[3];
+                if (UPWD && f < 4)
+                    upz[f] = C.up[2][0] * q[f][3] + C.up[2][1] * S[1] + C.up[2][2] * S[2] + C.up[2][3] * S[3];
+            }
+            CellDerivs<T> d;
+            d.lr = q[0][3];
+            d.glr[0] = o.lr_dx;
+            d.glr[1] = o.lr_dy;
+            d.glr[2] = dz[0];
+            d.laplr = o.lr_lap + dzz[0];
+            const T auz = ac_abs(q[3][3]);
+            d.upl = UPWD ? o.lr_up + auz * upz[0] : T(0);
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                d.u[i] = q[1 + i][3];
+                d.gu[i][0] = o.u_dx[i];
+                d.gu[i][1] = o.u_dy[i];
+                d.gu[i][2] = dz[1 + i];
+                d.lapu[i] = o.u_lap[i] + dzz[1 + i];
+                d.upu[i] = UPWD ? o.u_up[i] + auz * upz[1 + i] : T(0);
+                d.lapa[i] = o.a_lap[i] + dzz[4 + i];
+            }
+            d.gdu[0] = ch[0][0];
+            d.gdu[1] = ch[1][0];
+            d.gdu[2] = ch[2][0] + dzz[3];
+            d.gda[0] = ch[3][0];
+            d.gda[1] = ch[4][0];
+            d.gda[2] = ch[5][0] + dzz[6];
+            d.ga[0][0] = T(0);
+            d.ga[0][1] = o.ax_dy;
+            d.ga[0][2] = dz[4];
+            d.ga[1][0] = o.ay_dx;
+            d.ga[1][1] = T(0);
+            d.ga[1][2] = dz[5];
+            d.ga[2][0] = o.az_dx;
+            d.ga[2][1] = o.az_dy;
+            d.ga[2][2] = T(0);
+            d.s = q[7][3];
+            d.gs[0] = o.s_dx;
+            d.gs[1] = o.s_dy;
+            d.gs[2] = dz[7];
+            d.laps = o.s_lap + dzz[7];
+
+            T rhs[NF];
+            mhd_rhs<T, UPWD, ENTROPY>(d, a.P, rhs);
+            const long long c = (long long)(gx + R) + sy * (gy + R) + sz * (long long)(z + R);
+            T fnew[NF];
+#pragma unroll
+            for (int f = 0; f < NF; ++f) {
+                if (f == 7 && !ENTROPY) {
+                    fnew[f] = q[f][3];
+                    a.out[f][c] = q[f][3];
+                    continue;
+                }
+                rk_update<T, KB>(a, f, c, q[f][3], rhs[f], dt, &fnew[f]);
+                a.out[f][c] = fnew[f];
+            }
+            if (a.step == 2) {
+                const double u2 = (double)fnew[1] * (double)fnew[1] + (double)fnew[2] * (double)fnew[2] +
+                                  (double)fnew[3] * (double)fnew[3];
+                u2max = fmax(u2max, u2);
+            }
+        }
+
+        // ---- (3) remaining in-plane derivatives of plane p
+        const T aux = ac_abs(q[1][6]), auy = ac_abs(q[2][6]);
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+            const T c0 = q[f][6];
+            T Dx[4], Dy[4], Sx[4], Sy[4];
+#pragma unroll
+            for (int i = 1; i <= 3; ++i) {
+                const T xp = L[f].at(i, 0), xm = L[f].at(-i, 0);
+                const T yp = L[f].at(0, i), ym = L[f].at(0, -i);
+                Dx[i] = xp - xm;
+                Sx[i] = xp + xm;
+                Dy[i] = yp - ym;
+                Sy[i] = yp + ym;
+            }
+            const T dxx = C.d2[0][0] * c0 + C.d2[0][1] * Sx[1] + C.d2[0][2] * Sx[2] + C.d2[0][3] * Sx[3];
+            const T dyy = C.d2[1][0] * c0 + C.d2[1][1] * Sy[1] + C.d2[1][2] * Sy[2] + C.d2[1][3] * Sy[3];
+            T upxy = T(0);
+            if (UPWD && f < 4) {
+                const T upx = C.up[0][0] * c0 + C.up[0][1] * Sx[1] + C.up[0][2] * Sx[2] + C.up[0][3] * Sx[3];
+                const T upy = C.up[1][0] * c0 + C.up[1][1] * Sy[1] + C.up[1][2] * Sy[2] + C.up[1][3] * Sy[3];
+                upxy = aux * upx + auy * upy;
+            }
+            const bool need_dx = (f == 0 || f == 2 || f == 5 || f == 7);
+            const bool need_dy = (f == 0 || f == 1 || f == 4 || f == 7);
+            T dx = T(0), dy = T(0);
+            if (need_dx)
+                dx = C.d1[0][1] * Dx[1] + C.d1[0][2] * Dx[2] + C.d1[0][3] * Dx[3];
+            if (need_dy)
+                dy = C.d1[1][1] * Dy[1] + C.d1[1][2] * Dy[2] + C.d1[1][3] * Dy[3];
+            if (f == 0) {
+                nv.lr_dx = dx;
+                nv.lr_dy = dy;
+                nv.lr_lap = dxx + dyy;
+                nv.lr_up = upxy;
+            }
+            else if (f < 4) {
+                if (f == 1) nv.u_dy[0] = dy;
+                if (f == 2) nv.u_dx[1] = dx;
+                nv.u_lap[f - 1] = dxx + dyy;
+                nv.u_up[f - 1] = upxy;
+            }
+            else if (f < 7) {
+                if (f == 4) nv.ax_dy = dy;
+                if (f == 5) nv.ay_dx = dx;
+                nv.a_lap[f - 4] = dxx + dyy;
+            }
+            else {
+                nv.s_dx = dx;
+                nv.s_dy = dy;
+                nv.s_lap = dxx + dyy;
+            }
+            // in-plane parts of grad(div v): x gets d_xx v_x, y gets d_yy v_y
+            if (f == 1) ch[0][3] += dxx;
+            if (f == 2) ch[1][3] += dyy;
+            if (f == 4) ch[3][3] += dxx;
+            if (f == 5) ch[4][3] += dyy;
+        }
+        // xy-mixed derivatives of u_x, u_y, A_x, A_y
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+            const int fx = 1 + 3 * v, fy = 2 + 3 * v;
+            T mxx = T(0), myy = T(0);
+#pragma unroll
+            for (int i = 1; i <= 3; ++i) {
+                mxx += C.mx[0][i] * ((L[fx].at(i, i) + L[fx].at(-i, -i)) - (L[fx].at(i, -i) + L[fx].at(-i, i)));
+                myy += C.mx[0][i] * ((L[fy].at(i, i) + L[fy].at(-i, -i)) - (L[fy].at(i, -i) + L[fy].at(-i, i)));
+            }
+            ch[3 * v + 0][3] += myy;  // d_xy v_y into (grad div v)_x
+            ch[3 * v + 1][3] += mxx;  // d_xy v_x into (grad div v)_y
+        }
+
+        // ---- (4) roll the register state by one plane
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+#pragma unroll
+            for (int i = 0; i < 6; ++i)
+                q[f][i] = q[f][i + 1];
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+#pragma unroll
+            for (int i = 0; i < 6; ++i)
+                ch[c][i] = ch[c][i + 1];
+            ch[c][6] = T(0);
+        }
+        dly[0] = dly[1];
+        dly[1] = dly[2];
+        dly[2] = dly[3];
+
+        // plane p + 1 has landed in the other buffer and every wave is done with this one
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+    }
+    if (a.step == 2 && a.umax2_next != nullptr) {
+        wave_max_to_global(u2max, a.umax2_next);
+        if (a.time_acc != nullptr && blockIdx.x == 0 && tid == 0)
+            *a.time_acc += (double)dt;
+    }
+}
+
+// Launch shapes (TX, TY, ZC). TX = 64 makes one wave per tile row.
+static const TiledShape kShapes[] = {
+    {64, 4, 32}, {64, 4, 64}, {64, 4, 128}, {64, 4, 256},
+    {32, 8, 64}, {32, 8, 128}, {64, 2, 64}, {64, 2, 128},
+};
+
+int tiled_num_shapes() { return (int)(sizeof(kShapes) / sizeof(kShapes[0])); }
+TiledShape tiled_shape(int i) { return kShapes[i]; }
+
+bool tiled_shape_fits(TiledShape s, int nx, int ny, int nz)
+{
+    bool known = false;
+    for (const TiledShape& k : kShapes)
+        known = known || (k.tx == s.tx && k.ty == s.ty);
+    return known && s.zchunk > 0 && nx % s.tx == 0 && ny % s.ty == 0 && nz % s.zchunk == 0;
+}
+
+template <typename T, bool UPWD, bool ENTROPY, int KB, int TX, int TY>
+static void launch_shape(const SolveArgs<T>& a, int zchunk, hipStream_t s)
+{
+    const int ntx = a.nx / TX, nty = a.ny / TY, nch = a.nz / zchunk;
+    const dim3 tpb(TX, TY, 1);
+    const dim3 bpg((unsigned)(ntx * nty * nch), 1, 1);
+    hipLaunchKernelGGL((solve_tiled_kernel<T, UPWD, ENTROPY, KB, TX, TY>), bpg, tpb, 0, s, a,
+                       zchunk, ntx, nty, nch);
+}
+
+template <typename T, bool UPWD, bool ENTROPY, int KB>
+static bool launch_kb(const SolveArgs<T>& a, TiledShape sh, hipStream_t s)
+{
+    if (sh.tx == 64 && sh.ty == 4) launch_shape<T, UPWD, ENTROPY, KB, 64, 4>(a, sh.zchunk, s);
+    else if (sh.tx == 32 && sh.ty == 8) launch_shape<T, UPWD, ENTROPY, KB, 32, 8>(a, sh.zchunk, s);
+    else if (sh.tx == 64 && sh.ty == 2) launch_shape<T, UPWD, ENTROPY, KB, 64, 2>(a, sh.zchunk, s);
+    else return false;
+    return true;
+}
 
 template <typename T>
-bool launch_solve_tiled(const SolveArgs<T>&, bool, bool, int, TiledShape, hipStream_t)
+bool launch_solve_tiled(const SolveArgs<T>& a, bool upwd, bool entropy, int kb, TiledShape sh,
+                        hipStream_t s)
 {
-    return false;
+    if (!tiled_shape_fits(sh, a.nx, a.ny, a.nz))
+        return false;
+#define AC_TILED_KB(U, E)                                                                   \
+    do {                                                                                    \
+        if (kb == 0) return launch_kb<T, U, E, 0>(a, sh, s);                                \
+        if (kb == 8) return launch_kb<T, U, E, (sizeof(T) == 4 ? 8 : 0)>(a, sh, s);         \
+        if (kb == 16) return launch_kb<T, U, E, (sizeof(T) == 4 ? 16 : 0)>(a, sh, s);       \
+        return launch_kb<T, U, E, (sizeof(T) == 4 ? 32 : 0)>(a, sh, s);                     \
+    } while (0)
+    if (upwd && entropy) AC_TILED_KB(true, true);
+    if (upwd) AC_TILED_KB(true, false);
+    if (entropy) AC_TILED_KB(false, true);
+    AC_TILED_KB(false, false);
+#undef AC_TILED_KB
 }
 
 template bool launch_solve_tiled<AcRealT>(const SolveArgs<AcRealT>&, bool, bool, int, TiledShape,

@@ -1,0 +1,118 @@
+"""GPU parity of the LDS-tiled 2.5D solve kernel: against the oracle at small sizes for every
+launch shape, and against the direct (one cell per thread) kernel at larger sizes, through the C
+ABI (launch shape selected with AC_solve_tile_x / AC_solve_tile_y / AC_solve_zchunk)."""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+from astaroth_amd import astaroth as ac
+import oracle.oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def physics_info(lib, n, shape=(0, 0, 0), **over):
+    p = dict(AC_len=(2 * np.pi,) * 3, AC_cs_sound=1.0, AC_gamma=5.0 / 3.0, AC_nu_visc=5e-3,
+             AC_eta=5e-3, AC_chi=5e-3, AC_K_heatcond=5e-3, AC_lupwd=True, AC_lentropy=True,
+             AC_solve_tile_x=shape[0], AC_solve_tile_y=shape[1], AC_solve_zchunk=shape[2])
+    p.update(over)
+    return lib.config(n, **p)
+
+
+def initial(lib, info, seed=2024, perturb=True):
+    mesh = ac.HostMesh(lib, info)
+    assert lib.acHostMeshInitFourier(ctypes.byref(mesh.struct), seed, 16, 4, 0.05, 0.0) == 0
+    data = mesh.data.copy()
+    if perturb:
+        rng = np.random.default_rng(seed)
+        data[0] += 0.01 * rng.standard_normal(data[0].shape).astype(data.dtype)
+        data[7] += 0.01 * rng.standard_normal(data[7].shape).astype(data.dtype)
+        O.periodic_fill(data, O.config_from_info(info))
+    return data
+
+
+def run(lib, info, data, dt, steps, expect_shape=None):
+    mesh = ac.HostMesh(lib, info, data)
+    with ac.Device(lib, info) as dev:
+        if expect_shape is not None:
+            sh = (ctypes.c_int(), ctypes.c_int(), ctypes.c_int())
+            lib.acDeviceGetSolveLaunchShape(dev.handle, *[ctypes.byref(v) for v in sh])
+            assert tuple(v.value for v in sh) == tuple(expect_shape)
+        dev.load_mesh(mesh)
+        for _ in range(steps):
+            dev.step(dt)
+        dev.periodic()
+        out = ac.HostMesh(lib, info)
+        dev.store_mesh(out)
+    return out.data
+
+
+@pytest.fixture(scope="module", params=["fp32", "fp64"])
+def lib(request, require_gpu):
+    return ac.load(request.param)
+
+
+@pytest.mark.parametrize("n,shape", [((64, 32, 32), (64, 4, 32)), ((32, 32, 64), (32, 8, 64)),
+                                     ((64, 16, 64), (64, 2, 64))])
+@pytest.mark.parametrize("upwd,entropy", [(True, True), (False, False)])
+def test_tiled_vs_oracle(lib, n, shape, upwd, entropy):
+    say(f"case tiled vs oracle {lib.precision} n={n} shape={shape} upwd={upwd} entropy={entropy}")
+    info = physics_info(lib, n, shape, AC_lupwd=upwd, AC_lentropy=entropy)
+    data = initial(lib, info)
+    dt, steps = 2e-3, 3
+    cand = run(lib, info, data, dt, steps, expect_shape=shape)
+    model = data.astype(np.float64)
+    cfg = O.config_from_info(info)
+    for _ in range(steps):
+        O.integrate_step(model, cfg, dt)
+    O.periodic_fill(model, cfg)
+    tol = 2e-5 if lib.precision == "fp32" else 1e-12
+    for q, e in enumerate(O.field_errors(model, cand)):
+        say(f"  {ac.FIELD_NAMES[q]}: maxrel={e['maxrel']:.3g}")
+        assert e["maxrel"] < tol, (q, e)
+
+
+@pytest.mark.parametrize("shape", [(64, 4, 64), (64, 4, 128), (32, 8, 128)])
+def test_tiled_vs_direct_128(lib, shape):
+    say(f"case tiled vs direct 128^3 {lib.precision} shape={shape}")
+    n = (128, 128, 128)
+    info_t = physics_info(lib, n, shape)
+    info_d = physics_info(lib, n, (-1, 0, 0))  # tile_x < 0 pins the direct kernel
+    data = initial(lib, info_t, seed=5)
+    dt, steps = 1e-2, 2
+    a = run(lib, info_t, data, dt, steps, expect_shape=shape)
+    b = run(lib, info_d, data, dt, steps, expect_shape=(0, 0, 0))
+    tol = 2e-6 if lib.precision == "fp32" else 1e-13
+    for q, e in enumerate(O.field_errors(b, a)):
+        say(f"  {ac.FIELD_NAMES[q]}: maxrel={e['maxrel']:.3g}")
+        assert e["maxrel"] < tol, (q, e)
+
+
+@pytest.mark.parametrize("kb", [8, 16, 32])
+def test_tiled_kahan_closer_to_fp64(kb, require_gpu):
+    """Kahan-compensated FP32 stays closer to the FP64 solution than plain FP32."""
+    say(f"case tiled kahan {kb}")
+    n = (64, 32, 32)
+    shape = (64, 4, 32)
+    l32, l64 = ac.load("fp32"), ac.load("fp64")
+    info64 = physics_info(l64, n, shape)
+    data = initial(l64, info64, seed=11)
+    dt, steps = 2e-3, 40
+    ref = run(l64, info64, data, dt, steps)
+    plain = run(l32, physics_info(l32, n, shape), data.astype(np.float32), np.float32(dt), steps)
+    comp = run(l32, physics_info(l32, n, shape, AC_kahan_bits=kb), data.astype(np.float32),
+               np.float32(dt), steps)
+    ep = O.field_errors(ref, plain)
+    ec = O.field_errors(ref, comp)
+    for q in range(8):
+        say(f"  {ac.FIELD_NAMES[q]}: plain {ep[q]['maxrel']:.3g} kahan{kb} {ec[q]['maxrel']:.3g}")
+    worst_p = max(e["maxrel"] for e in ep)
+    worst_c = max(e["maxrel"] for e in ec)
+    assert worst_c < worst_p
