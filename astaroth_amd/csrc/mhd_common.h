@@ -243,6 +243,81 @@ __device__ __forceinline__ void rk_update(const SolveArgs<T>& a, int q, long lon
     }
 }
 
+// The same update addressed as (uniform plane base, 32-bit in-plane offset): wp, cp and op
+// point at the cell's z plane of w, compensation and output, so every access can use a
+// scalar base register and one vector offset.
+template <typename T, int KB>
+__device__ __forceinline__ T rk_update_plane(T* __restrict__ wp, void* __restrict__ cp,
+                                             T* __restrict__ op, unsigned off, int step, T alpha,
+                                             T beta, T f, T rhs, T dt)
+{
+    T wn = dt * rhs;
+    if (step > 0)
+        wn = alpha * wp[off] + wn;
+    if (step < 2)
+        wp[off] = wn;
+    T fn;
+    if (KB == 0) {
+        fn = add_rn(f, mul_rn(beta, wn));
+    }
+    else {
+        const float cc = comp_load<KB>(cp, off, f);
+        const float y = beta * wn - cc;
+        const float t = f + y;
+        const float cn = (t - f) - y;
+        comp_store<KB>(cp, off, cn, t);
+        fn = t;
+    }
+    op[off] = fn;
+    return fn;
+}
+
+// Raw stored compensation of one value (float bits, or the sign-extended int8/int16).
+template <int KB>
+__device__ __forceinline__ int comp_raw_load(const void* base, unsigned off)
+{
+    if (KB == 32)
+        return __float_as_int(((const float*)base)[off]);
+    if (KB == 16)
+        return ((const int16_t*)base)[off];
+    return ((const int8_t*)base)[off];
+}
+template <int KB>
+__device__ __forceinline__ float comp_decode(int raw, float f)
+{
+    if (KB == 32)
+        return __int_as_float(raw);
+    return ldexpf((float)raw, ulp_exp_f32(f) - KB);
+}
+
+// As rk_update_plane, with w of the cell (wold; ignored at step 0) and, for KB > 0, the raw
+// compensation (craw) already loaded, so their latency hides behind the stencil work.
+template <typename T, int KB>
+__device__ __forceinline__ T rk_update_pre(T* __restrict__ wp, void* __restrict__ cp,
+                                           T* __restrict__ op, unsigned off, int step, T alpha,
+                                           T beta, T f, T rhs, T dt, T wold, int craw)
+{
+    T wn = dt * rhs;
+    if (step > 0)
+        wn = alpha * wold + wn;
+    if (step < 2)
+        wp[off] = wn;
+    T fn;
+    if (KB == 0) {
+        fn = add_rn(f, mul_rn(beta, wn));
+    }
+    else {
+        const float cc = comp_decode<KB>(craw, f);
+        const float y = beta * wn - cc;
+        const float t = f + y;
+        const float cn = (t - f) - y;
+        comp_store<KB>(cp, off, cn, t);
+        fn = t;
+    }
+    op[off] = fn;
+    return fn;
+}
+
 template <typename T>
 __device__ __forceinline__ T solve_dt(const SolveArgs<T>& a)
 {

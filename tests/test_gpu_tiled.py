@@ -59,10 +59,20 @@ def lib(request, require_gpu):
     return ac.load(request.param)
 
 
+RING = (32, 16)  # five-plane ring kernel, FP32 only
+
+
+def skip_ring_fp64(lib, shape):
+    if tuple(shape[:2]) == RING and lib.precision == "fp64":
+        pytest.skip("the ring kernel exists for FP32 only (its LDS ring does not fit FP64)")
+
+
 @pytest.mark.parametrize("n,shape", [((64, 32, 32), (64, 4, 32)), ((32, 32, 64), (32, 8, 64)),
-                                     ((64, 16, 64), (64, 2, 64))])
+                                     ((64, 16, 64), (64, 2, 64)), ((32, 32, 32), (32, 16, 32)),
+                                     ((64, 48, 64), (32, 16, 64))])
 @pytest.mark.parametrize("upwd,entropy", [(True, True), (False, False)])
 def test_tiled_vs_oracle(lib, n, shape, upwd, entropy):
+    skip_ring_fp64(lib, shape)
     say(f"case tiled vs oracle {lib.precision} n={n} shape={shape} upwd={upwd} entropy={entropy}")
     info = physics_info(lib, n, shape, AC_lupwd=upwd, AC_lentropy=entropy)
     data = initial(lib, info)
@@ -79,8 +89,9 @@ def test_tiled_vs_oracle(lib, n, shape, upwd, entropy):
         assert e["maxrel"] < tol, (q, e)
 
 
-@pytest.mark.parametrize("shape", [(64, 4, 64), (64, 4, 128), (32, 8, 128)])
+@pytest.mark.parametrize("shape", [(64, 4, 64), (64, 4, 128), (32, 8, 128), (32, 16, 64)])
 def test_tiled_vs_direct_128(lib, shape):
+    skip_ring_fp64(lib, shape)
     say(f"case tiled vs direct 128^3 {lib.precision} shape={shape}")
     n = (128, 128, 128)
     info_t = physics_info(lib, n, shape)
@@ -89,18 +100,20 @@ def test_tiled_vs_direct_128(lib, shape):
     dt, steps = 1e-2, 2
     a = run(lib, info_t, data, dt, steps, expect_shape=shape)
     b = run(lib, info_d, data, dt, steps, expect_shape=(0, 0, 0))
-    tol = 2e-6 if lib.precision == "fp32" else 1e-13
+    # Two FP32 kernels with different summation orders, on data with grid-scale noise in lnrho
+    # and ss: ~1e-6 of the field scale apart; an index or coefficient slip is >= 1e-3.
+    tol = 1e-5 if lib.precision == "fp32" else 1e-13
     for q, e in enumerate(O.field_errors(b, a)):
         say(f"  {ac.FIELD_NAMES[q]}: maxrel={e['maxrel']:.3g}")
         assert e["maxrel"] < tol, (q, e)
 
 
-@pytest.mark.parametrize("kb", [8, 16, 32])
-def test_tiled_kahan_closer_to_fp64(kb, require_gpu):
+@pytest.mark.parametrize("kb,shape", [(8, (64, 4, 32)), (16, (64, 4, 32)), (32, (64, 4, 32)),
+                                      (8, (32, 16, 32)), (16, (32, 16, 32))])
+def test_tiled_kahan_closer_to_fp64(kb, shape, require_gpu):
     """Kahan-compensated FP32 stays closer to the FP64 solution than plain FP32."""
-    say(f"case tiled kahan {kb}")
+    say(f"case tiled kahan {kb} shape={shape}")
     n = (64, 32, 32)
-    shape = (64, 4, 32)
     l32, l64 = ac.load("fp32"), ac.load("fp64")
     info64 = physics_info(l64, n, shape)
     data = initial(l64, info64, seed=11)
