@@ -52,11 +52,13 @@ def per_launch_bytes(n3, real_bytes, kahan_bits):
     return n3 * algorithmic_bytes_per_cell_update(real_bytes, kahan_bits) / 3.0
 
 
-def make_device(ac, prec, n, kahan_bits=0):
+def make_device(ac, prec, n, kahan_bits=0, shape=(0, 0, 0)):
     lib = ac.load(prec)
     info = lib.config((n, n, n), AC_len=(2 * np.pi,) * 3, AC_cs_sound=1.0, AC_gamma=5.0 / 3.0,
                       AC_nu_visc=5e-3, AC_eta=5e-3, AC_chi=5e-3, AC_K_heatcond=5e-3,
-                      AC_lupwd=True, AC_lentropy=True, AC_kahan_bits=kahan_bits)
+                      AC_lupwd=True, AC_lentropy=True, AC_kahan_bits=kahan_bits,
+                      AC_solve_tile_x=shape[0], AC_solve_tile_y=shape[1],
+                      AC_solve_zchunk=shape[2])
     mesh = ac.HostMesh(lib, info)
     ac.check(lib.acHostMeshInitFourier(ctypes.byref(mesh.struct), 2024, 16, 4, 0.05, 0.0),
              "acHostMeshInitFourier")
@@ -116,6 +118,22 @@ def cpu_baseline(n=96, min_seconds=10.0, max_steps=40):
                        f"({dt:.1f} s) on {threads} host threads")
 
 
+def copy_calibration(nbytes):
+    """One device-to-device copy of nbytes (a plain copy kernel of known traffic: nbytes read,
+    nbytes written), so FETCH_SIZE / WRITE_SIZE counters can be scaled to bytes."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) or \
+            hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def load_traffic(path):
     try:
         with open(path) as fp:
@@ -133,6 +151,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--variant-steps", type=int, default=30)
+    ap.add_argument("--shape", type=int, nargs=3, default=(0, 0, 0),
+                    help="pin the solve launch shape TX TY ZC (0 0 0 = autotune)")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="after the run, copy this many bytes device-to-device (counter scale)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -143,7 +165,7 @@ def main():
     from astaroth_amd import astaroth as ac
     n = args.n
     n3 = n ** 3
-    lib, dev = make_device(ac, "fp32", n, 0)
+    lib, dev = make_device(ac, "fp32", n, 0, tuple(args.shape))
     wall, solve_ms, launches = timed(lib, dev, args.steps, args.warmup)
     tx, ty, tz = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     lib.acDeviceGetSolveLaunchShape(dev.handle, ctypes.byref(tx), ctypes.byref(ty),
@@ -214,6 +236,8 @@ def main():
                     out[f"fp32_kahan{kb}_over_fp64"] = v / fp64
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
     print(json.dumps(out), flush=True)
 
 
