@@ -87,6 +87,7 @@ EXPORTED = [
     "acHostMeshInitFourier", "acGetRealSize", "acGetMeshInfoSize", "acGetMeshSize",
     "acGetBuildInfo",
     "acAdjointWorkspaceBytes", "acDeviceIntegrateSubstepTangent", "acDeviceIntegrateSubstepAdjoint",
+    "acDeviceLoadMeshFromDevicePtr", "acDeviceStoreMeshToDevicePtr",
 ]
 
 
@@ -165,6 +166,22 @@ def _make_types(c_real):
     return Real3, AcMeshInfo, AcMesh
 
 
+def _bind_hip_runtime():
+    """PyTorch-ROCm ships its own libamdhip64 (SONAME libamdhip64.so.7) and loads it by its
+    unversioned file name. If this library were loaded first, the dynamic linker would bind
+    /opt/rocm's runtime to it and torch would later load a second HIP runtime into the process,
+    which finds no GPU. Importing torch first makes the library's libamdhip64.so.7 dependency
+    resolve to the runtime torch already holds: one runtime per process, so device pointers
+    and streams pass freely between torch tensors and this library. Without torch installed
+    the system runtime is used."""
+    if os.environ.get("ASTAROTH_SYSTEM_HIP_RUNTIME") == "1":
+        return
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 class AstarothLib:
     """One precision build of the library (AcReal = float for fp32, double for fp64)."""
 
@@ -177,6 +194,7 @@ class AstarothLib:
                 f"{path} is missing: build the HIP libraries first (python -m astaroth_amd.build)")
         self.path = path
         self.precision = precision
+        _bind_hip_runtime()
         self.lib = ctypes.CDLL(path)
         self.c_real = ctypes.c_float if precision == "fp32" else ctypes.c_double
         self.np_real = np.float32 if precision == "fp32" else np.float64
@@ -255,8 +273,9 @@ class AstarothLib:
             "acDeviceIntegrateSubstepTangent": [dev, i, i, R, ctypes.c_void_p, ctypes.c_void_p,
                                                 ctypes.c_void_p, ctypes.c_void_p],
             "acDeviceIntegrateSubstepAdjoint": [dev, i, i, R, ctypes.c_void_p, ctypes.c_void_p,
-                                                ctypes.c_void_p, ctypes.c_void_p,
-                                                ctypes.c_void_p],
+                                                ctypes.c_void_p, ctypes.c_void_p],
+            "acDeviceLoadMeshFromDevicePtr": [dev, i, ctypes.c_void_p],
+            "acDeviceStoreMeshToDevicePtr": [dev, i, ctypes.c_void_p],
         }
         for name, args in sig.items():
             if not hasattr(L, name):

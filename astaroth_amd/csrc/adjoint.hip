@@ -1,2 +1,495 @@
-// adjoint.hip -- tangent-linear and adjoint kernels of one RK3 substep (placeholder).
+// adjoint.hip -- tangent-linear and adjoint of one fused RK3 substep (no reference counterpart).
+//
+// Forward substep s (per field q, cell c):  w' = alpha_s w + dt R(f),  f' = f + beta_s w'.
+// Tangent:  dw' = alpha_s dw + dt (J(f) df + dR/dnu dnu + dR/deta deta),  df' = df + beta_s dw'.
+// Adjoint:  mu = lw' + beta_s lf',  lw = alpha_s mu,  lf = lf' + dt J(f)^T mu,
+//           lnu += dt <mu, dR/dnu>,  leta += dt <mu, dR/deta>.
+// J^T mu is computed in three launches: (A) per cell, the adjoint of every derivative quantity
+// the right-hand side reads (55 fields, "derivative adjoints"); (B) their periodic halo fill,
+// which is the transpose of the forward halo copy; (C) per cell, the transposed stencils
+// gathered from the derivative adjoints (first derivatives are antisymmetric, so their
+// transpose is the negative; second, upwind and mixed ones are symmetric).
+#include <cstdio>
+#include <cstring>
+
 #include "kernels.h"
+#include "mhd_adjoint.h"
+
+namespace acamd {
+
+constexpr int NG = 55;  // derivative-adjoint fields
+// layout of the derivative adjoints
+constexpr int G_LR = 0;                       // val, D1 x y z, lap, up x y z
+constexpr int G_U = 8;                        // + 8 i: val, D1 x y z, lap, up x y z
+constexpr int G_GDU = 32;                     // grad(div u) x y z
+constexpr int G_A = 35;                       // + 4 i: D1 x y z, lap
+constexpr int G_GDA = 47;                     // grad(div A) x y z
+constexpr int G_S = 50;                       // val, D1 x y z, lap
+
+template <typename T>
+struct AdjArgs {
+    SolveArgs<T> s;        // f = s.in (halos filled), coefficients, parameters, alpha, beta
+    T* lf;                 // adjoint of f, NF slabs of m^3, updated in place
+    T* lw;                 // adjoint of w, updated in place
+    T* g;                  // NG slabs of m^3 (derivative adjoints)
+    double* partials;      // 2 per block (nu, eta)
+    const T* df_in;        // tangent: df (halos filled)
+    T* df_out;             // tangent: df'
+    T* dw;                 // tangent: dw, updated in place
+    T dnu, deta;
+    long long count;       // m^3
+};
+
+template <typename T>
+__device__ __forceinline__ T d1(const T* __restrict__ f, long long c, long long s, const T* k)
+{
+    return k[1] * (f[c + s] - f[c - s]) + k[2] * (f[c + 2 * s] - f[c - 2 * s]) +
+           k[3] * (f[c + 3 * s] - f[c - 3 * s]);
+}
+template <typename T>
+__device__ __forceinline__ T d2(const T* __restrict__ f, long long c, long long s, const T* k)
+{
+    return k[0] * f[c] + k[1] * (f[c + s] + f[c - s]) + k[2] * (f[c + 2 * s] + f[c - 2 * s]) +
+           k[3] * (f[c + 3 * s] + f[c - 3 * s]);
+}
+template <typename T>
+__device__ __forceinline__ T dm(const T* __restrict__ f, long long c, long long sa, long long sb,
+                                const T* k)
+{
+    T r = T(0);
+#pragma unroll
+    for (int o = 1; o <= 3; ++o)
+        r += k[o] * ((f[c + o * sa + o * sb] + f[c - o * sa - o * sb]) -
+                     (f[c + o * sa - o * sb] + f[c - o * sa + o * sb]));
+    return r;
+}
+
+// All derivative quantities of the eight fields at cell c (field q at base + q * count).
+template <typename T, bool UPWD>
+__device__ __forceinline__ void gather(const T* const* f, long long c, long long sy, long long sz,
+                                       const Coeffs<T>& C, CellDerivs<T>& d, UpwindDirs<T>& up)
+{
+    const long long st[3] = {1, sy, sz};
+    T val[NF], grad[NF][3], lap[NF];
+#pragma unroll
+    for (int q = 0; q < NF; ++q) {
+        val[q] = f[q][c];
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax)
+            grad[q][ax] = d1(f[q], c, st[ax], C.d1[ax]);
+        lap[q] = d2(f[q], c, st[0], C.d2[0]) + d2(f[q], c, st[1], C.d2[1]) + d2(f[q], c, st[2], C.d2[2]);
+    }
+    d.lr = val[0];
+    d.laplr = lap[0];
+    d.s = val[7];
+    d.laps = lap[7];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        d.glr[ax] = grad[0][ax];
+        d.gs[ax] = grad[7][ax];
+        d.u[ax] = val[1 + ax];
+        d.lapu[ax] = lap[1 + ax];
+        d.lapa[ax] = lap[4 + ax];
+#pragma unroll
+        for (int bx = 0; bx < 3; ++bx) {
+            d.gu[ax][bx] = grad[1 + ax][bx];
+            d.ga[ax][bx] = grad[4 + ax][bx];
+        }
+    }
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        up.ul[ax] = UPWD ? d2(f[0], c, st[ax], C.up[ax]) : T(0);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+            up.uu[i][ax] = UPWD ? d2(f[1 + i], c, st[ax], C.up[ax]) : T(0);
+    }
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+        const T* fx = f[1 + 3 * v];
+        const T* fy = f[2 + 3 * v];
+        const T* fz = f[3 + 3 * v];
+        T* gd = v == 0 ? d.gdu : d.gda;
+        gd[0] = d2(fx, c, st[0], C.d2[0]) + dm(fy, c, st[0], st[1], C.mx[0]) + dm(fz, c, st[0], st[2], C.mx[1]);
+        gd[1] = dm(fx, c, st[0], st[1], C.mx[0]) + d2(fy, c, st[1], C.d2[1]) + dm(fz, c, st[1], st[2], C.mx[2]);
+        gd[2] = dm(fx, c, st[0], st[2], C.mx[1]) + dm(fy, c, st[1], st[2], C.mx[2]) + d2(fz, c, st[2], C.d2[2]);
+    }
+    combine_upwind(d, up);
+}
+
+template <typename T>
+__device__ __forceinline__ void cell_index(const SolveArgs<T>& a, bool& active, long long& c)
+{
+    const int i = threadIdx.x + blockIdx.x * blockDim.x;
+    const int j = threadIdx.y + blockIdx.y * blockDim.y;
+    const int k = blockIdx.z;
+    active = i < a.nx && j < a.ny && k < a.nz;
+    c = (long long)(i + R) + (long long)a.mx * (j + R) + a.sz * (k + R);
+}
+
+// ---------------------------------------------------------------- tangent
+template <typename T, bool UPWD, bool ENTROPY>
+__global__ void __launch_bounds__(256) tangent_kernel(AdjArgs<T> A)
+{
+    const SolveArgs<T>& a = A.s;
+    bool active;
+    long long c;
+    cell_index(a, active, c);
+    if (!active)
+        return;
+    CellDerivs<T> d, dd;
+    UpwindDirs<T> up, dup;
+    gather<T, UPWD>(a.in, c, a.mx, a.sz, a.C, d, up);
+    const T* df[NF];
+#pragma unroll
+    for (int q = 0; q < NF; ++q)
+        df[q] = A.df_in + (long long)q * A.count;
+    gather<T, UPWD>(df, c, a.mx, a.sz, a.C, dd, dup);
+    T rdot[NF];
+    mhd_rhs_jvp<T, UPWD, ENTROPY>(d, up, dd, dup, a.P, A.dnu, A.deta, rdot);
+#pragma unroll
+    for (int q = 0; q < NF; ++q) {
+        const long long o = (long long)q * A.count + c;
+        if (q == 7 && !ENTROPY) {
+            A.df_out[o] = df[q][c];
+            continue;
+        }
+        T dwn = a.dt * rdot[q];
+        if (a.step > 0)
+            dwn = a.alpha * A.dw[o] + dwn;
+        if (a.step < 2)
+            A.dw[o] = dwn;
+        A.df_out[o] = df[q][c] + a.beta * dwn;
+    }
+}
+
+// ---------------------------------------------------------------- adjoint (A): local part
+template <typename T, bool UPWD, bool ENTROPY>
+__global__ void __launch_bounds__(256) adjoint_local_kernel(AdjArgs<T> A)
+{
+    __shared__ double sh[2][4];
+    const SolveArgs<T>& a = A.s;
+    bool active;
+    long long c;
+    cell_index(a, active, c);
+    double pnu = 0.0, peta = 0.0;
+    if (active) {
+        CellDerivs<T> d, ad;
+        UpwindDirs<T> up, aup;
+        gather<T, UPWD>(a.in, c, a.mx, a.sz, a.C, d, up);
+        T mu[NF];
+#pragma unroll
+        for (int q = 0; q < NF; ++q) {
+            const long long o = (long long)q * A.count + c;
+            mu[q] = A.lw[o] + a.beta * A.lf[o];
+        }
+        if (!ENTROPY)
+            mu[7] = T(0);
+        T anu, aeta;
+        mhd_rhs_vjp<T, UPWD, ENTROPY>(d, up, mu, a.P, ad, aup, anu, aeta);
+        pnu = (double)a.dt * (double)anu;
+        peta = (double)a.dt * (double)aeta;
+        T* g = A.g;
+        const long long n = A.count;
+        auto put = [&](int k, T v) { g[(long long)k * n + c] = v; };
+        put(G_LR + 0, ad.lr);
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            put(G_LR + 1 + ax, ad.glr[ax]);
+            put(G_LR + 5 + ax, aup.ul[ax]);
+        }
+        put(G_LR + 4, ad.laplr);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            put(G_U + 8 * i + 0, ad.u[i]);
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) {
+                put(G_U + 8 * i + 1 + ax, ad.gu[i][ax]);
+                put(G_U + 8 * i + 5 + ax, aup.uu[i][ax]);
+                put(G_A + 4 * i + ax, ad.ga[i][ax]);
+            }
+            put(G_U + 8 * i + 4, ad.lapu[i]);
+            put(G_A + 4 * i + 3, ad.lapa[i]);
+            put(G_GDU + i, ad.gdu[i]);
+            put(G_GDA + i, ad.gda[i]);
+            put(G_S + 1 + i, ad.gs[i]);
+        }
+        put(G_S + 0, ad.s);
+        put(G_S + 4, ad.laps);
+    }
+    // block sums of the parameter sensitivities (FP64)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        pnu += __shfl_xor(pnu, off, 64);
+        peta += __shfl_xor(peta, off, 64);
+    }
+    const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+    if ((tid & 63) == 0) {
+        sh[0][tid >> 6] = pnu;
+        sh[1][tid >> 6] = peta;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const int nw = (blockDim.x * blockDim.y + 63) >> 6;
+        double s0 = 0.0, s1 = 0.0;
+        for (int w = 0; w < nw; ++w) {
+            s0 += sh[0][w];
+            s1 += sh[1][w];
+        }
+        const long long b = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
+        A.partials[2 * b + 0] = s0;
+        A.partials[2 * b + 1] = s1;
+    }
+}
+
+__global__ void __launch_bounds__(256) adjoint_param_reduce(const double* partials, long long nb,
+                                                           double* out)
+{
+    __shared__ double sh[2][4];
+    double s0 = 0.0, s1 = 0.0;
+    for (long long t = threadIdx.x; t < nb; t += blockDim.x) {
+        s0 += partials[2 * t];
+        s1 += partials[2 * t + 1];
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        s0 += __shfl_xor(s0, off, 64);
+        s1 += __shfl_xor(s1, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][threadIdx.x >> 6] = s0;
+        sh[1][threadIdx.x >> 6] = s1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t0 = 0.0, t1 = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+            t0 += sh[0][w];
+            t1 += sh[1][w];
+        }
+        out[0] += t0;
+        out[1] += t1;
+    }
+}
+
+// ---------------------------------------------------------------- adjoint (C): transposed stencils
+template <typename T>
+__device__ __forceinline__ T scalar_transpose(const T* g, long long n, int base, long long c,
+                                              const long long* st, const Coeffs<T>& C, bool has_val,
+                                              bool has_up)
+{
+    // val - sum_a D1_a(g_1a) + sum_a D2_a(g_lap) + sum_a U_a(g_up_a)
+    T r = has_val ? g[(long long)base * n + c] : T(0);
+    const T* gl = g + (long long)(base + 4) * n;
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        r -= d1(g + (long long)(base + 1 + ax) * n, c, st[ax], C.d1[ax]);
+        r += d2(gl, c, st[ax], C.d2[ax]);
+        if (has_up)
+            r += d2(g + (long long)(base + 5 + ax) * n, c, st[ax], C.up[ax]);
+    }
+    return r;
+}
+
+template <typename T, bool UPWD, bool ENTROPY>
+__global__ void __launch_bounds__(256) adjoint_transpose_kernel(AdjArgs<T> A)
+{
+    const SolveArgs<T>& a = A.s;
+    bool active;
+    long long c;
+    cell_index(a, active, c);
+    if (!active)
+        return;
+    const long long n = A.count;
+    const long long st[3] = {1, (long long)a.mx, a.sz};
+    const T* g = A.g;
+    const Coeffs<T>& C = a.C;
+    T tr[NF];
+    tr[0] = scalar_transpose(g, n, G_LR, c, st, C, true, UPWD);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        tr[1 + i] = scalar_transpose(g, n, G_U + 8 * i, c, st, C, true, UPWD);
+    // A fields: D1 (3) and lap only
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        T r = T(0);
+        const T* gl = g + (long long)(G_A + 4 * i + 3) * n;
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            r -= d1(g + (long long)(G_A + 4 * i + ax) * n, c, st[ax], C.d1[ax]);
+            r += d2(gl, c, st[ax], C.d2[ax]);
+        }
+        tr[4 + i] = r;
+    }
+    // ss: val, D1, lap (base layout differs: val, D1 x y z, lap)
+    {
+        T r = g[(long long)G_S * n + c];
+        const T* gl = g + (long long)(G_S + 4) * n;
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            r -= d1(g + (long long)(G_S + 1 + ax) * n, c, st[ax], C.d1[ax]);
+            r += d2(gl, c, st[ax], C.d2[ax]);
+        }
+        tr[7] = r;
+    }
+    // grad(div v): forward gd_i = sum_j S_ij v_j with S_ii = D2_i, S_ij = M_ij (symmetric)
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+        const int gb = v == 0 ? G_GDU : G_GDA;
+        const T* G0 = g + (long long)(gb + 0) * n;
+        const T* G1 = g + (long long)(gb + 1) * n;
+        const T* G2 = g + (long long)(gb + 2) * n;
+        tr[1 + 3 * v] += d2(G0, c, st[0], C.d2[0]) + dm(G1, c, st[0], st[1], C.mx[0]) + dm(G2, c, st[0], st[2], C.mx[1]);
+        tr[2 + 3 * v] += dm(G0, c, st[0], st[1], C.mx[0]) + d2(G1, c, st[1], C.d2[1]) + dm(G2, c, st[1], st[2], C.mx[2]);
+        tr[3 + 3 * v] += dm(G0, c, st[0], st[2], C.mx[1]) + dm(G1, c, st[1], st[2], C.mx[2]) + d2(G2, c, st[2], C.d2[2]);
+    }
+#pragma unroll
+    for (int q = 0; q < NF; ++q) {
+        const long long o = (long long)q * n + c;
+        if (q == 7 && !ENTROPY) {
+            A.lw[o] = T(0);  // f'_s = f_s: lf passes through unchanged
+            continue;
+        }
+        const T mu = A.lw[o] + a.beta * A.lf[o];
+        A.lf[o] = A.lf[o] + a.dt * tr[q];
+        A.lw[o] = a.alpha * mu;
+    }
+}
+
+// ---------------------------------------------------------------- launchers
+template <typename T>
+static dim3 grid_of(const SolveArgs<T>& a)
+{
+    return dim3((a.nx + 63) / 64, (a.ny + 3) / 4, a.nz);
+}
+
+template <typename T>
+void launch_tangent(const AdjArgs<T>& A, bool upwd, bool entropy, hipStream_t s)
+{
+    const dim3 tpb(64, 4, 1), bpg = grid_of(A.s);
+    if (upwd && entropy) hipLaunchKernelGGL((tangent_kernel<T, true, true>), bpg, tpb, 0, s, A);
+    else if (upwd) hipLaunchKernelGGL((tangent_kernel<T, true, false>), bpg, tpb, 0, s, A);
+    else if (entropy) hipLaunchKernelGGL((tangent_kernel<T, false, true>), bpg, tpb, 0, s, A);
+    else hipLaunchKernelGGL((tangent_kernel<T, false, false>), bpg, tpb, 0, s, A);
+}
+
+long long adjoint_num_blocks(int nx, int ny, int nz)
+{
+    return (long long)((nx + 63) / 64) * ((ny + 3) / 4) * nz;
+}
+
+template <typename T>
+void launch_adjoint(const AdjArgs<T>& A, bool upwd, bool entropy, double* param_grad,
+                    hipStream_t s)
+{
+    const dim3 tpb(64, 4, 1), bpg = grid_of(A.s);
+#define AC_ADJ(K)                                                                                  \
+    do {                                                                                           \
+        if (upwd && entropy) hipLaunchKernelGGL((K<T, true, true>), bpg, tpb, 0, s, A);            \
+        else if (upwd) hipLaunchKernelGGL((K<T, true, false>), bpg, tpb, 0, s, A);                 \
+        else if (entropy) hipLaunchKernelGGL((K<T, false, true>), bpg, tpb, 0, s, A);              \
+        else hipLaunchKernelGGL((K<T, false, false>), bpg, tpb, 0, s, A);                          \
+    } while (0)
+    AC_ADJ(adjoint_local_kernel);
+    const long long nb = (long long)bpg.x * bpg.y * bpg.z;
+    hipLaunchKernelGGL(adjoint_param_reduce, dim3(1), dim3(256), 0, s, (const double*)A.partials, nb,
+                       param_grad);
+    // (B) periodic fill of the derivative adjoints, eight fields per launch
+    for (int k0 = 0; k0 < NG; k0 += NF) {
+        T* ptrs[NF];
+        const int cnt = NG - k0 < NF ? NG - k0 : NF;
+        for (int q = 0; q < cnt; ++q)
+            ptrs[q] = A.g + (long long)(k0 + q) * A.count;
+        launch_periodic_fill<T>(ptrs, cnt, A.s.nx, A.s.ny, A.s.nz, s);
+    }
+    AC_ADJ(adjoint_transpose_kernel);
+#undef AC_ADJ
+}
+
+int adjoint_num_fields() { return NG; }
+
+template void launch_tangent<AcRealT>(const AdjArgs<AcRealT>&, bool, bool, hipStream_t);
+template void launch_adjoint<AcRealT>(const AdjArgs<AcRealT>&, bool, bool, double*, hipStream_t);
+
+}  // namespace acamd
+
+// ------------------------------------------------------------------ C entry points
+#include "../../include/astaroth.h"
+
+namespace acamd {
+// defined in device.hip
+void device_solve_args(Device d, SolveArgs<AcRealT>& a, int step, AcRealT dt);
+hipStream_t device_stream(Device d, Stream s);
+int device_id(Device d);
+}  // namespace acamd
+
+using namespace acamd;
+
+extern "C" size_t acAdjointWorkspaceBytes(const Device d)
+{
+    const AcMeshInfo info = acDeviceGetLocalConfig(d);
+    const int3 m = info.int3_params[AC_mlocal];
+    const int3 n = info.int3_params[AC_nlocal];
+    const size_t count = (size_t)m.x * m.y * m.z;
+    const size_t g = count * NG * sizeof(AcReal);
+    const size_t partials = (size_t)adjoint_num_blocks(n.x, n.y, n.z) * 2 * sizeof(double);
+    return ((g + 255) / 256) * 256 + partials;
+}
+
+extern "C" AcResult acDeviceIntegrateSubstepTangent(const Device d, const Stream stream,
+                                                    const int step_number, const AcReal dt,
+                                                    const AcReal* df_in, AcReal* df_out,
+                                                    AcReal* dw, const double* dparams)
+{
+    if (step_number < 0 || step_number > 2) {
+        fprintf(stderr, "astaroth_amd: acDeviceIntegrateSubstepTangent: step_number must be 0..2\n");
+        return AC_FAILURE;
+    }
+    if (hipSetDevice(device_id(d)) != hipSuccess)
+        return AC_FAILURE;
+    const AcMeshInfo info = acDeviceGetLocalConfig(d);
+    const int3 m = info.int3_params[AC_mlocal];
+    AdjArgs<AcReal> A;
+    memset(&A, 0, sizeof(A));
+    device_solve_args(d, A.s, step_number, dt);
+    A.count = (long long)m.x * m.y * m.z;
+    A.df_in = df_in;
+    A.df_out = df_out;
+    A.dw = dw;
+    A.dnu = dparams ? (AcReal)dparams[0] : (AcReal)0;
+    A.deta = dparams ? (AcReal)dparams[1] : (AcReal)0;
+    hipStream_t s = device_stream(d, stream);
+    AcReal* ptrs[NF];
+    for (int q = 0; q < NF; ++q)
+        ptrs[q] = const_cast<AcReal*>(df_in) + (long long)q * A.count;
+    launch_periodic_fill<AcReal>(ptrs, NF, A.s.nx, A.s.ny, A.s.nz, s);
+    launch_tangent<AcReal>(A, info.bool_params[AC_lupwd] != 0, info.bool_params[AC_lentropy] != 0, s);
+    return hipGetLastError() == hipSuccess ? AC_SUCCESS : AC_FAILURE;
+}
+
+extern "C" AcResult acDeviceIntegrateSubstepAdjoint(const Device d, const Stream stream,
+                                                    const int step_number, const AcReal dt,
+                                                    AcReal* lam_f, AcReal* lam_w, void* workspace,
+                                                    double* param_grad)
+{
+    if (step_number < 0 || step_number > 2) {
+        fprintf(stderr, "astaroth_amd: acDeviceIntegrateSubstepAdjoint: step_number must be 0..2\n");
+        return AC_FAILURE;
+    }
+    if (hipSetDevice(device_id(d)) != hipSuccess)
+        return AC_FAILURE;
+    const AcMeshInfo info = acDeviceGetLocalConfig(d);
+    const int3 m = info.int3_params[AC_mlocal];
+    AdjArgs<AcReal> A;
+    memset(&A, 0, sizeof(A));
+    device_solve_args(d, A.s, step_number, dt);
+    A.count = (long long)m.x * m.y * m.z;
+    A.lf = lam_f;
+    A.lw = lam_w;
+    A.g = (AcReal*)workspace;
+    const size_t gbytes = ((size_t)A.count * NG * sizeof(AcReal) + 255) / 256 * 256;
+    A.partials = (double*)((char*)workspace + gbytes);
+    hipStream_t s = device_stream(d, stream);
+    launch_adjoint<AcReal>(A, info.bool_params[AC_lupwd] != 0, info.bool_params[AC_lentropy] != 0,
+                           param_grad, s);
+    return hipGetLastError() == hipSuccess ? AC_SUCCESS : AC_FAILURE;
+}

@@ -1049,3 +1049,36 @@ extern "C" AcResult acDeviceAutotune(const Device d, const Stream stream)
     d->shape_tuned = false;
     return autotune(d, get_stream(d, stream));
 }
+
+// ------------------------------------------------------------------ helpers for adjoint.hip
+namespace acamd {
+void device_solve_args(Device d, SolveArgs<AcReal>& a, int step, AcReal dt)
+{
+    fill_solve_args(d, a, step, dt);
+}
+hipStream_t device_stream(Device d, Stream s) { return get_stream(d, s); }
+int device_id(Device d) { return d->id; }
+}  // namespace acamd
+
+extern "C" AcResult acDeviceLoadMeshFromDevicePtr(const Device d, const Stream stream,
+                                                  const AcReal* src)
+{
+    AC_HIP_CHECK(hipSetDevice(d->id));
+    hipStream_t s = get_stream(d, stream);
+    for (int q = 0; q < NF; ++q)
+        AC_HIP_CHECK(hipMemcpyAsync(d->in[q], src + (size_t)q * d->count, d->count * sizeof(AcReal),
+                                    hipMemcpyDeviceToDevice, s));
+    if (d->kahan_bits > 0)
+        launch_zero_bytes(d->slab_comp, d->count * (size_t)(d->kahan_bits / 8) * NF, s);
+    return AC_SUCCESS;
+}
+
+extern "C" AcResult acDeviceStoreMeshToDevicePtr(const Device d, const Stream stream, AcReal* dst)
+{
+    AC_HIP_CHECK(hipSetDevice(d->id));
+    hipStream_t s = get_stream(d, stream);
+    for (int q = 0; q < NF; ++q)
+        AC_HIP_CHECK(hipMemcpyAsync(dst + (size_t)q * d->count, d->in[q], d->count * sizeof(AcReal),
+                                    hipMemcpyDeviceToDevice, s));
+    return AC_SUCCESS;
+}

@@ -17,7 +17,11 @@ Printed: one JSON line with the headline metric plus
                    against the 8 TB/s HBM peak; `traffic` from committed rocprofv3 counters;
   cpu_baseline  -- the FP64 two-pass OpenMP CPU restatement of the reference's CPU build
                    timed on this host on a bounded sample (rank 0, N = 1 only);
-  variants      -- FP64 and Kahan-compensated FP32 cell-updates/s on the same GPU.
+  variants      -- FP64 and Kahan-compensated FP32 cell-updates/s on the same GPU;
+  adjoint       -- time of one adjoint RK3 step (three adjoint substeps with their halo fills)
+                   and of one checkpointed reverse step (plus recomputing the step's first two
+                   substeps and moving states), each against one forward step, at the same
+                   grid and fixed dt.
 """
 import argparse
 import ctypes
@@ -92,7 +96,7 @@ def timed(lib, dev, steps, warmup, courant=0.2):
     return t1 - t0, tot.value, cnt.value
 
 
-def cpu_baseline(n=96, min_seconds=10.0, max_steps=40):
+def cpu_baseline(n=96, min_seconds=10.0, max_steps=1000):
     """ref-cpu: the reference's CPU-build algorithm (two-pass RK3, FP64), restated in C++ with
     OpenMP over z on every host core. Bounded sample: n^3 cells, steps until min_seconds."""
     import oracle.oracle as O
@@ -142,6 +146,60 @@ def load_traffic(path):
         return None
 
 
+def adjoint_cost(ac, n, steps=5):
+    """Wall time per step, device-synchronised, of forward steps, adjoint steps and full
+    reverse steps of astaroth_amd.adjoint.MHDAdjoint on the benchmark state (FP32)."""
+    import torch
+    from astaroth_amd import adjoint as AD
+    lib = ac.load("fp32")
+    info = lib.config((n, n, n), AC_len=(2 * np.pi,) * 3, AC_cs_sound=1.0, AC_gamma=5.0 / 3.0,
+                      AC_nu_visc=5e-3, AC_eta=5e-3, AC_chi=5e-3, AC_K_heatcond=5e-3,
+                      AC_lupwd=True, AC_lentropy=True)
+    mesh = ac.HostMesh(lib, info)
+    ac.check(lib.acHostMeshInitFourier(ctypes.byref(mesh.struct), 2024, 16, 4, 0.05, 0.0),
+             "acHostMeshInitFourier")
+    dt = 1e-3
+    runner = AD.MHDAdjoint(lib, info, steps, dt)
+    try:
+        runner.dev.load_mesh(mesh)
+        del mesh
+        runner._sync()
+        lam_f = 1e-3 * torch.randn(runner.shape, dtype=runner.tdtype, device=runner.tdev)
+        lam_w = torch.zeros_like(lam_f)
+        torch.cuda.synchronize(runner.tdev)
+
+        def clock(fn):
+            fn()  # warm-up (first launches, autotuning)
+            runner._sync()
+            torch.cuda.synchronize(runner.tdev)
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                fn()
+            runner._sync()
+            torch.cuda.synchronize(runner.tdev)
+            return (time.perf_counter() - t0) / steps * 1e3
+
+        def adjoint_step():
+            for s in (2, 1, 0):
+                runner.dev.periodic()
+                ac.check(lib.acDeviceIntegrateSubstepAdjoint(
+                    runner.dev.handle, 0, s, lib.np_real(dt), ctypes.c_void_p(lam_f.data_ptr()),
+                    ctypes.c_void_p(lam_w.data_ptr()), ctypes.c_void_p(runner.workspace.data_ptr()),
+                    ctypes.c_void_p(runner.param_grad.data_ptr())), "adjoint substep")
+
+        fwd = clock(lambda: runner.advance(1))
+        adj = clock(adjoint_step)
+        rev = clock(lambda: runner.reverse_step(lam_f, lam_w))
+        return {"grid": [n, n, n], "dtype": "f32", "steps": steps,
+                "forward_ms_per_step": round(fwd, 3), "adjoint_ms_per_step": round(adj, 3),
+                "reverse_ms_per_step": round(rev, 3),
+                "adjoint_over_forward": round(adj / fwd, 3),
+                "reverse_over_forward": round(rev / fwd, 3),
+                "workspace_bytes": int(runner.workspace.numel())}
+    finally:
+        runner.close()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,6 +209,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--variant-steps", type=int, default=30)
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--shape", type=int, nargs=3, default=(0, 0, 0),
                     help="pin the solve launch shape TX TY ZC (0 0 0 = autotune)")
     ap.add_argument("--copy-calibration", type=int, default=0,
@@ -234,6 +293,11 @@ def main():
                 v = variants.get(f"fp32_kahan{kb}", {}).get("cell_updates_per_s")
                 if v:
                     out[f"fp32_kahan{kb}_over_fp64"] = v / fp64
+    if not args.no_adjoint:
+        try:
+            out["adjoint"] = adjoint_cost(ac, n)
+        except Exception as e:  # report, do not hide
+            out["adjoint"] = {"error": str(e)}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     if args.copy_calibration:

@@ -407,6 +407,29 @@ AcResult acDeviceGetSolveTiming(const Device device, double* total_ms, int* laun
 /* The hipStream_t behind a Stream (for interop with other HIP code; default priority). */
 AcResult acDeviceGetStreamHandle(const Device device, const Stream stream, void** hip_stream);
 
+/* Copies the whole state (all fields, m^3 each, field q at src + q m^3) from / to device memory
+ * the caller owns (interop with other device code; no reference counterpart). */
+AcResult acDeviceLoadMeshFromDevicePtr(const Device device, const Stream stream, const AcReal* src);
+AcResult acDeviceStoreMeshToDevicePtr(const Device device, const Stream stream, AcReal* dst);
+
+/* ---- Differentiation of one RK3 substep (no reference counterpart) ----
+ * Linearisation point: the device's current state `in` (halos filled by the caller, as before
+ * acDeviceIntegrateSubstep). Slabs hold NUM_VTXBUF_HANDLES fields of m^3 values each.
+ * Tangent:  dw' = alpha_s dw + dt (J df + dR/dnu dnu + dR/deta deta),  df' = df + beta_s dw'.
+ *   df_in: tangent of f (its halos are filled by the call), df_out: df', dw: updated in place,
+ *   dparams: host pointer to {dnu, deta} (NULL = zero).
+ * Adjoint:  mu = lw + beta_s lf,  lf <- lf + dt J^T mu,  lw <- alpha_s mu (both in place);
+ *   param_grad: device pointer to two doubles, += dt <mu, dR/dnu>, dt <mu, dR/deta>.
+ *   workspace: device memory of acAdjointWorkspaceBytes(device) bytes. */
+size_t acAdjointWorkspaceBytes(const Device device);
+AcResult acDeviceIntegrateSubstepTangent(const Device device, const Stream stream,
+                                         const int step_number, const AcReal dt,
+                                         const AcReal* df_in, AcReal* df_out, AcReal* dw,
+                                         const double* dparams);
+AcResult acDeviceIntegrateSubstepAdjoint(const Device device, const Stream stream,
+                                         const int step_number, const AcReal dt, AcReal* lam_f,
+                                         AcReal* lam_w, void* workspace, double* param_grad);
+
 /* ===================== Host interface (ref include/astaroth_helpers.h, astaroth_utils.h) ===== */
 
 /* ref astaroth_helpers.h:291, src/core/update_params/update_params.cc:19-142 */
