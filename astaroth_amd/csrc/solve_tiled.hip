@@ -58,8 +58,13 @@ __global__ void __launch_bounds__(TX* TY) solve_tiled_kernel(SolveArgs<T> a, int
     constexpr int LPW = LP * WPE;                // dwords per field-plane
     constexpr int NCHUNK = (LPW + 63) / 64;      // wave-wide DMA chunks (one dword per lane)
     constexpr int NR = (NCHUNK + NW - 1) / NW;   // chunks per wave per field-plane
+    // Each field-plane occupies NR * NW whole chunks in LDS, so every wave issues exactly NR
+    // DMA instructions per field with all lanes active (no exec-mask branches): lanes past the
+    // plane's last dword read a valid address and land in the padding.
+    constexpr int LPS = NR * NW * 64 / WPE;      // LDS stride of one field-plane (elements)
     static_assert(NT % 64 == 0, "workgroup must be whole waves");
-    __shared__ T lds[2][NF * LP];
+    static_assert((NR * NW * 64) % WPE == 0, "field-plane stride");
+    __shared__ T lds[2][NF * LPS];
 
     // ---- XCD-aware tile numbering: workgroup b runs on XCD b % 8; give each XCD a contiguous
     // range of tiles (x fastest, then y, then z-chunk) so halo rows are shared within one L2.
@@ -82,12 +87,12 @@ __global__ void __launch_bounds__(TX* TY) solve_tiled_kernel(SolveArgs<T> a, int
     // ---- DMA slots of this lane: chunk j = wave + r * NW covers the dwords [64 j, 64 j + 64) of
     // a field-plane; lane l moves dword d = 64 j + l, i.e. word d % WPE of element d / WPE.
     int g_off[NR];  // byte offset of that dword within the array plane
-    bool g_ok[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
         const int j = wave + r * NW;
-        const int d = j * 64 + lane;
-        g_ok[r] = (j < NCHUNK) && (d < LPW);
+        int d = j * 64 + lane;
+        if (d >= LPW)
+            d = 0;  // padding lane: re-read the first dword (in bounds, value unused)
         const int e = d / WPE, wd = d - e * WPE;
         const int ly = e / LX, lx = e - ly * LX;
         g_off[r] = ((x0 + lx) + (int)sy * (y0 + ly)) * (int)sizeof(T) + 4 * wd;
@@ -98,13 +103,12 @@ __global__ void __launch_bounds__(TX* TY) solve_tiled_kernel(SolveArgs<T> a, int
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
             const char* src = (const char*)(a.in[f] + base);
-            char* dst = (char*)&lds[b][f * LP];
+            char* dst = (char*)&lds[b][f * LPS];
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
                 const int j = wave + r * NW;
-                if (j < NCHUNK && g_ok[r])  // j is wave-uniform; g_ok masks the tail lanes
-                    __builtin_amdgcn_global_load_lds((const void*)(src + g_off[r]),
-                                                     (lds_ptr_t)(dst + 256 * j), 4, 0, 0);
+                __builtin_amdgcn_global_load_lds((const void*)(src + g_off[r]),
+                                                 (lds_ptr_t)(dst + 256 * j), 4, 0, 0);
             }
         }
     };
@@ -167,7 +171,7 @@ __global__ void __launch_bounds__(TX* TY) solve_tiled_kernel(SolveArgs<T> a, int
         Lds<T, LX> L[NF];
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
-            L[f].p = &lds[cur][f * LP + ctr];
+            L[f].p = &lds[cur][f * LPS + ctr];
             q[f][6] = L[f].at(0, 0);
         }
         PlaneVals<T>& nv = dly[3];
