@@ -241,15 +241,13 @@ __global__ void __launch_bounds__(256) adjoint_local_kernel(AdjArgs<T> A)
     }
 }
 
-__global__ void __launch_bounds__(256) adjoint_param_reduce(const double* partials, long long nb,
-                                                           double* out)
+// Deterministic two-stage sum of the (nu, eta) block partials: stage 1 reduces contiguous
+// ranges of partials per workgroup, stage 2 (one workgroup) adds the range sums into out.
+constexpr int PARAM_RANGES = 256;
+
+__device__ __forceinline__ void block_sum2(double& s0, double& s1)
 {
     __shared__ double sh[2][4];
-    double s0 = 0.0, s1 = 0.0;
-    for (long long t = threadIdx.x; t < nb; t += blockDim.x) {
-        s0 += partials[2 * t];
-        s1 += partials[2 * t + 1];
-    }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         s0 += __shfl_xor(s0, off, 64);
@@ -266,8 +264,40 @@ __global__ void __launch_bounds__(256) adjoint_param_reduce(const double* partia
             t0 += sh[0][w];
             t1 += sh[1][w];
         }
-        out[0] += t0;
-        out[1] += t1;
+        s0 = t0;
+        s1 = t1;
+    }
+}
+
+__global__ void __launch_bounds__(256) adjoint_param_stage1(const double* partials, long long nb,
+                                                            double* ranges)
+{
+    const long long per = (nb + PARAM_RANGES - 1) / PARAM_RANGES;
+    const long long b0 = per * blockIdx.x;
+    const long long b1 = b0 + per < nb ? b0 + per : nb;
+    double s0 = 0.0, s1 = 0.0;
+    for (long long t = b0 + threadIdx.x; t < b1; t += blockDim.x) {
+        s0 += partials[2 * t];
+        s1 += partials[2 * t + 1];
+    }
+    block_sum2(s0, s1);
+    if (threadIdx.x == 0) {
+        ranges[2 * blockIdx.x] = s0;
+        ranges[2 * blockIdx.x + 1] = s1;
+    }
+}
+
+__global__ void __launch_bounds__(256) adjoint_param_stage2(const double* ranges, double* out)
+{
+    double s0 = 0.0, s1 = 0.0;
+    for (int t = threadIdx.x; t < PARAM_RANGES; t += blockDim.x) {
+        s0 += ranges[2 * t];
+        s1 += ranges[2 * t + 1];
+    }
+    block_sum2(s0, s1);
+    if (threadIdx.x == 0) {
+        out[0] += s0;
+        out[1] += s1;
     }
 }
 
@@ -391,7 +421,10 @@ void launch_adjoint(const AdjArgs<T>& A, bool upwd, bool entropy, double* param_
     } while (0)
     AC_ADJ(adjoint_local_kernel);
     const long long nb = (long long)bpg.x * bpg.y * bpg.z;
-    hipLaunchKernelGGL(adjoint_param_reduce, dim3(1), dim3(256), 0, s, (const double*)A.partials, nb,
+    double* ranges = A.partials + 2 * nb;
+    hipLaunchKernelGGL(adjoint_param_stage1, dim3(PARAM_RANGES), dim3(256), 0, s,
+                       (const double*)A.partials, nb, ranges);
+    hipLaunchKernelGGL(adjoint_param_stage2, dim3(1), dim3(256), 0, s, (const double*)ranges,
                        param_grad);
     // (B) periodic fill of the derivative adjoints, eight fields per launch
     for (int k0 = 0; k0 < NG; k0 += NF) {
@@ -431,7 +464,8 @@ extern "C" size_t acAdjointWorkspaceBytes(const Device d)
     const int3 n = info.int3_params[AC_nlocal];
     const size_t count = (size_t)m.x * m.y * m.z;
     const size_t g = count * NG * sizeof(AcReal);
-    const size_t partials = (size_t)adjoint_num_blocks(n.x, n.y, n.z) * 2 * sizeof(double);
+    const size_t partials =
+        ((size_t)adjoint_num_blocks(n.x, n.y, n.z) + PARAM_RANGES) * 2 * sizeof(double);
     return ((g + 255) / 256) * 256 + partials;
 }
 

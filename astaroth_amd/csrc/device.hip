@@ -161,6 +161,7 @@ static void fill_solve_args(Device d, SolveArgs<AcReal>& a, int step, AcReal dt)
     a.my = n.y + 2 * R;
     a.mz = n.z + 2 * R;
     a.sz = (long long)a.mx * a.my;
+    a.fstride = (long long)d->count;
     make_coeffs(info, a.C);
     make_phys(info, a.P);
     const AcReal alpha[3] = {(AcReal)0.0, (AcReal)(-5.0 / 9.0), (AcReal)(-153.0 / 128.0)};

@@ -35,13 +35,37 @@ void launch_solve_direct(const SolveArgs<T>& a, bool upwd, bool entropy, int kah
                          hipStream_t stream);
 
 // LDS-tiled 2.5D solve. Returns false when the launch shape does not fit the grid.
+// Tiles of 64 x 4 and 64 x 2 run the delay-line kernel (solve_tiled.hip); tiles of 32 x 8,
+// 16 x 16 and 32 x 16 the four-plane ring kernel (solve_ring.hip).
 struct TiledShape {
     int tx, ty, zchunk;
 };
 template <typename T>
 bool launch_solve_tiled(const SolveArgs<T>& a, bool upwd, bool entropy, int kahan_bits,
                         TiledShape shape, hipStream_t stream);
-// Candidate launch shapes of the tiled kernel (for the autotuner).
+template <typename T>
+bool launch_solve_ring(const SolveArgs<T>& a, bool upwd, bool entropy, int kahan_bits,
+                       TiledShape shape, hipStream_t stream);
+// LDS pointer type of the global->LDS DMA builtin
+typedef __attribute__((address_space(3))) void* lds_ptr_t;
+
+// Returns v unchanged, but opaque to the optimiser. Used on per-lane 32-bit offsets inside the
+// plane loops: otherwise loop-invariant code motion folds them into 64-bit per-lane addresses
+// of every field array (a VGPR pair per field and array) that stay live across the loop, where
+// a uniform 64-bit base in SGPRs plus the 32-bit offset is what the memory instructions take.
+__device__ __forceinline__ unsigned opaque(unsigned v)
+{
+    asm volatile("" : "+v"(v));
+    return v;
+}
+// The same for a wave-uniform 64-bit value kept in a scalar register pair: recomputing
+// per-field bases from it each plane step beats holding eight bases per array across the loop.
+__device__ __forceinline__ long long opaque_s64(long long v)
+{
+    asm volatile("" : "+s"(v));
+    return v;
+}
+// Candidate launch shapes of the tiled kernels (for the autotuner).
 int tiled_num_shapes();
 TiledShape tiled_shape(int i);
 bool tiled_shape_fits(TiledShape s, int nx, int ny, int nz);

@@ -47,6 +47,9 @@ struct SolveArgs {
     int nx, ny, nz;    // interior extent
     int mx, my, mz;    // extent with halos
     long long sz;      // mx * my
+    // Elements between consecutive fields of in, out and w (each role is one slab; the tiled
+    // kernels address field f as slab + f * fstride and require that layout)
+    long long fstride;
     Coeffs<T> C;
     PhysParams<T> P;
     T dt;              // used when dt_dev == nullptr
@@ -272,15 +275,48 @@ __device__ __forceinline__ T rk_update_plane(T* __restrict__ wp, void* __restric
     return fn;
 }
 
-// Raw stored compensation of one value (float bits, or the sign-extended int8/int16).
-template <int KB>
-__device__ __forceinline__ int comp_raw_load(const void* base, unsigned off)
+// Access at (uniform base, 32-bit byte offset): the form global loads and stores take as one
+// scalar base register pair plus one vector offset, with no per-lane 64-bit address math.
+template <typename T>
+__device__ __forceinline__ T ld_b(const T* base, unsigned boff)
 {
+    return *(const T*)((const char*)base + boff);
+}
+template <typename T>
+__device__ __forceinline__ void st_b(T* base, unsigned boff, T v)
+{
+    *(T*)((char*)base + boff) = v;
+}
+
+// Raw stored compensation of one value (float bits, or the sign-extended int8/int16);
+// cboff is the byte offset in the compensation array.
+template <int KB>
+__device__ __forceinline__ int comp_raw_load(const void* base, unsigned cboff)
+{
+    const char* p = (const char*)base + cboff;
     if (KB == 32)
-        return __float_as_int(((const float*)base)[off]);
+        return __float_as_int(*(const float*)p);
     if (KB == 16)
-        return ((const int16_t*)base)[off];
-    return ((const int8_t*)base)[off];
+        return *(const int16_t*)p;
+    return *(const int8_t*)p;
+}
+template <int KB>
+__device__ __forceinline__ void comp_store_b(void* base, unsigned cboff, float comp, float f)
+{
+    char* p = (char*)base + cboff;
+    if (KB == 32) {
+        *(float*)p = comp;
+    }
+    else if (KB == 16) {
+        float q = rintf(ldexpf(comp, 16 - ulp_exp_f32(f)));
+        q = fminf(fmaxf(q, -32767.f), 32767.f);
+        *(int16_t*)p = (int16_t)q;
+    }
+    else {
+        float q = rintf(ldexpf(comp, 8 - ulp_exp_f32(f)));
+        q = fminf(fmaxf(q, -127.f), 127.f);
+        *(int8_t*)p = (int8_t)q;
+    }
 }
 template <int KB>
 __device__ __forceinline__ float comp_decode(int raw, float f)
@@ -291,17 +327,18 @@ __device__ __forceinline__ float comp_decode(int raw, float f)
 }
 
 // As rk_update_plane, with w of the cell (wold; ignored at step 0) and, for KB > 0, the raw
-// compensation (craw) already loaded, so their latency hides behind the stencil work.
+// compensation (craw) already loaded, so their latency hides behind the stencil work. boff is
+// the cell's byte offset in the plane of w and f (the compensation's follows from it).
 template <typename T, int KB>
 __device__ __forceinline__ T rk_update_pre(T* __restrict__ wp, void* __restrict__ cp,
-                                           T* __restrict__ op, unsigned off, int step, T alpha,
+                                           T* __restrict__ op, unsigned boff, int step, T alpha,
                                            T beta, T f, T rhs, T dt, T wold, int craw)
 {
     T wn = dt * rhs;
     if (step > 0)
         wn = alpha * wold + wn;
     if (step < 2)
-        wp[off] = wn;
+        st_b(wp, boff, wn);
     T fn;
     if (KB == 0) {
         fn = add_rn(f, mul_rn(beta, wn));
@@ -311,10 +348,10 @@ __device__ __forceinline__ T rk_update_pre(T* __restrict__ wp, void* __restrict_
         const float y = beta * wn - cc;
         const float t = f + y;
         const float cn = (t - f) - y;
-        comp_store<KB>(cp, off, cn, t);
+        comp_store_b<KB>(cp, boff / (unsigned)sizeof(T) * (unsigned)(KB / 8), cn, t);
         fn = t;
     }
-    op[off] = fn;
+    st_b(op, boff, fn);
     return fn;
 }
 

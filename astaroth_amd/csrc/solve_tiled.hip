@@ -25,14 +25,16 @@
 
 namespace acamd {
 
-// In-plane quantities of one plane that the output needs three planes later.
+// In-plane quantities of one plane that the output needs three planes later. The lnrho and u
+// ones ride a register delay line; the ten of A and ss (ax_dy, ay_dx, az_dx, az_dy, a_lap[3],
+// s_dx, s_dy, s_lap, in that order) a per-thread three-slot ring in LDS, which keeps the
+// register footprint within the budget of two waves per SIMD.
 template <typename T>
 struct PlaneVals {
     T lr_dx, lr_dy, lr_lap, lr_up;       // lnrho: d_x, d_y, d_xx + d_yy, |ux| up_x + |uy| up_y
     T u_dx[3], u_dy[3], u_lap[3], u_up[3];
-    T ax_dy, ay_dx, az_dx, az_dy, a_lap[3];
-    T s_dx, s_dy, s_lap;
 };
+constexpr int NDL = 10;  // delayed values kept in LDS
 
 // Chains: 0..2 grad(div u) x, y, z ; 3..5 grad(div A) x, y, z.
 constexpr int NCH = 6;
@@ -43,11 +45,9 @@ struct Lds {
     __device__ __forceinline__ T at(int dx, int dy) const { return p[dy * LX + dx]; }
 };
 
-typedef __attribute__((address_space(3))) void* lds_ptr_t;
-
 template <typename T, bool UPWD, bool ENTROPY, int KB, int TX, int TY>
-__global__ void __launch_bounds__(TX* TY) solve_tiled_kernel(SolveArgs<T> a, int zchunk, int ntx,
-                                                             int nty, int nchunks)
+__global__ void __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY == 256 ? 2 : 1))
+    solve_tiled_kernel(SolveArgs<T> a, int zchunk, int ntx, int nty, int nchunks)
 {
     constexpr int LX = TX + 2 * R;
     constexpr int LY = TY + 2 * R;
@@ -65,6 +65,7 @@ __global__ void __launch_bounds__(TX* TY) solve_tiled_kernel(SolveArgs<T> a, int
     static_assert(NT % 64 == 0, "workgroup must be whole waves");
     static_assert((NR * NW * 64) % WPE == 0, "field-plane stride");
     __shared__ T lds[2][NF * LPS];
+    __shared__ T dl[3][NDL][NT];  // delayed A / ss values, [slot][value][thread]
 
     // ---- XCD-aware tile numbering: workgroup b runs on XCD b % 8; give each XCD a contiguous
     // range of tiles (x fastest, then y, then z-chunk) so halo rows are shared within one L2.
@@ -86,7 +87,7 @@ __global__ void __launch_bounds__(TX* TY) solve_tiled_kernel(SolveArgs<T> a, int
 
     // ---- DMA slots of this lane: chunk j = wave + r * NW covers the dwords [64 j, 64 j + 64) of
     // a field-plane; lane l moves dword d = 64 j + l, i.e. word d % WPE of element d / WPE.
-    int g_off[NR];  // byte offset of that dword within the array plane
+    unsigned g_off[NR];  // byte offset of that dword within the array plane
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
         const int j = wave + r * NW;
@@ -95,19 +96,25 @@ __global__ void __launch_bounds__(TX* TY) solve_tiled_kernel(SolveArgs<T> a, int
             d = 0;  // padding lane: re-read the first dword (in bounds, value unused)
         const int e = d / WPE, wd = d - e * WPE;
         const int ly = e / LX, lx = e - ly * LX;
-        g_off[r] = ((x0 + lx) + (int)sy * (y0 + ly)) * (int)sizeof(T) + 4 * wd;
+        g_off[r] = (unsigned)(((x0 + lx) + (int)sy * (y0 + ly)) * (int)sizeof(T) + 4 * wd);
     }
+    // Field f of a role lives at slab + f * fstride; the per-field bases are recomputed from a
+    // stride made opaque once per plane step (scalar work) rather than held across the loop.
+    const char* const in0 = (const char*)a.in[0];
+    char* const out0 = (char*)a.out[0];
+    char* const w0 = (char*)a.w[0];
+    char* const comp0 = (char*)a.comp[0];
     // Issue the DMA of interior plane p (array z index p + R) into buffer b.
-    auto dma_plane = [&](int p, int b) {
-        const long long base = sz * (long long)(p + R);
+    auto dma_plane = [&](int p, int b, long long fsb) {
+        const long long base = sz * (long long)(p + R) * (long long)sizeof(T);
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
-            const char* src = (const char*)(a.in[f] + base);
+            const char* src = in0 + (base + f * fsb);
             char* dst = (char*)&lds[b][f * LPS];
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
                 const int j = wave + r * NW;
-                __builtin_amdgcn_global_load_lds((const void*)(src + g_off[r]),
+                __builtin_amdgcn_global_load_lds((const void*)(src + opaque(g_off[r])),
                                                  (lds_ptr_t)(dst + 256 * j), 4, 0, 0);
             }
         }
@@ -117,6 +124,7 @@ __global__ void __launch_bounds__(TX* TY) solve_tiled_kernel(SolveArgs<T> a, int
     T q[NF][7];
     T ch[NCH][7];
     PlaneVals<T> dly[4];
+    int dslot = 0;  // ring slot of this step's plane p (and of the output plane p - 3)
 #pragma unroll
     for (int f = 0; f < NF; ++f)
 #pragma unroll
@@ -134,15 +142,17 @@ __global__ void __launch_bounds__(TX* TY) solve_tiled_kernel(SolveArgs<T> a, int
     const int ctr = (ty + R) * LX + (tx + R);
     const unsigned ioff = (unsigned)((gx + R) + (int)sy * (gy + R));  // in-plane offset
 
-    dma_plane(z0 - R, 0);
+    dma_plane(z0 - R, 0, a.fstride * (long long)sizeof(T));
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     const int niter = zchunk + 2 * R;
     for (int it = 0; it < niter; ++it) {
         const int p = z0 - R + it;
         const int cur = it & 1;
+        const unsigned iob = opaque(ioff * (unsigned)sizeof(T));  // byte offset in a plane
+        const long long fsb = opaque_s64(a.fstride * (long long)sizeof(T));  // field stride, bytes
         if (it + 1 < niter)
-            dma_plane(p + 1, cur ^ 1);
+            dma_plane(p + 1, cur ^ 1, fsb);
         // w of this iteration's output cell, loaded early so its latency hides behind the
         // stencil work (it is needed only by the RK update at the end)
         T wpre[NF];
@@ -159,12 +169,14 @@ __global__ void __launch_bounds__(TX* TY) solve_tiled_kernel(SolveArgs<T> a, int
             if (a.step > 0) {
 #pragma unroll
                 for (int f = 0; f < NF; ++f)
-                    wpre[f] = a.w[f][poff + ioff];
+                    wpre[f] = ld_b((const T*)(w0 + (f * fsb + poff * (long long)sizeof(T))), iob);
             }
             if (KB > 0) {
 #pragma unroll
                 for (int f = 0; f < NF; ++f)
-                    cpre[f] = comp_raw_load<KB>((const char*)a.comp[f] + poff * (KB / 8), ioff);
+                    cpre[f] = comp_raw_load<KB>(comp0 + (f * fsb + poff * (long long)sizeof(T)) /
+                                                            (long long)sizeof(T) * (KB / 8),
+                                                iob / (unsigned)sizeof(T) * (unsigned)(KB / 8));
             }
         }
 
@@ -175,6 +187,7 @@ __global__ void __launch_bounds__(TX* TY) solve_tiled_kernel(SolveArgs<T> a, int
             q[f][6] = L[f].at(0, 0);
         }
         PlaneVals<T>& nv = dly[3];
+        T az_dx_p = T(0), az_dy_p = T(0);  // of plane p, stored to the LDS ring below
 
         // ---- (1) z-chains: plane p adds +c_i D(p) to plane p - i and -c_i D(p) to plane p + i
 #pragma unroll
@@ -209,8 +222,8 @@ __global__ void __launch_bounds__(TX* TY) solve_tiled_kernel(SolveArgs<T> a, int
                 nv.u_dy[1] = dyy_;
             }
             else {
-                nv.az_dx = dxz;
-                nv.az_dy = dyz;
+                az_dx_p = dxz;
+                az_dy_p = dyz;
                 (void)dxx_;
                 (void)dyy_;
             }
@@ -220,6 +233,10 @@ __global__ void __launch_bounds__(TX* TY) solve_tiled_kernel(SolveArgs<T> a, int
         const int z = p - R;
         if (it >= 2 * R) {
             const PlaneVals<T>& o = dly[0];
+            T dv[NDL];  // the A / ss values of plane z, written three steps ago
+#pragma unroll
+            for (int k = 0; k < NDL; ++k)
+                dv[k] = dl[dslot][k][tid];
             T dz[NF], dzz[NF], upz[4];
 #pragma unroll
             for (int f = 0; f < NF; ++f) {
@@ -250,7 +267,7 @@ __global__ void __launch_bounds__(TX* TY) solve_tiled_kernel(SolveArgs<T> a, int
                 d.gu[i][2] = dz[1 + i];
                 d.lapu[i] = o.u_lap[i] + dzz[1 + i];
                 d.upu[i] = UPWD ? o.u_up[i] + auz * upz[1 + i] : T(0);
-                d.lapa[i] = o.a_lap[i] + dzz[4 + i];
+                d.lapa[i] = dv[4 + i] + dzz[4 + i];
             }
             d.gdu[0] = ch[0][0];
             d.gdu[1] = ch[1][0];
@@ -259,19 +276,19 @@ __global__ void __launch_bounds__(TX* TY) solve_tiled_kernel(SolveArgs<T> a, int
             d.gda[1] = ch[4][0];
             d.gda[2] = ch[5][0] + dzz[6];
             d.ga[0][0] = T(0);
-            d.ga[0][1] = o.ax_dy;
+            d.ga[0][1] = dv[0];
             d.ga[0][2] = dz[4];
-            d.ga[1][0] = o.ay_dx;
+            d.ga[1][0] = dv[1];
             d.ga[1][1] = T(0);
             d.ga[1][2] = dz[5];
-            d.ga[2][0] = o.az_dx;
-            d.ga[2][1] = o.az_dy;
+            d.ga[2][0] = dv[2];
+            d.ga[2][1] = dv[3];
             d.ga[2][2] = T(0);
             d.s = q[7][3];
-            d.gs[0] = o.s_dx;
-            d.gs[1] = o.s_dy;
+            d.gs[0] = dv[7];
+            d.gs[1] = dv[8];
             d.gs[2] = dz[7];
-            d.laps = o.s_lap + dzz[7];
+            d.laps = dv[9] + dzz[7];
 
             T rhs[NF];
             mhd_rhs<T, UPWD, ENTROPY>(d, a.P, rhs);
@@ -279,15 +296,16 @@ __global__ void __launch_bounds__(TX* TY) solve_tiled_kernel(SolveArgs<T> a, int
             T fnew[NF];
 #pragma unroll
             for (int f = 0; f < NF; ++f) {
-                T* op = a.out[f] + poff;
+                const long long fo = f * fsb + poff * (long long)sizeof(T);  // bytes
+                T* op = (T*)(out0 + fo);
                 if (f == 7 && !ENTROPY) {
                     fnew[f] = q[f][3];
-                    op[ioff] = q[f][3];
+                    st_b(op, iob, q[f][3]);
                     continue;
                 }
-                T* wp = a.w[f] + poff;
-                void* cp = KB ? (void*)((char*)a.comp[f] + poff * (KB / 8)) : nullptr;
-                fnew[f] = rk_update_pre<T, KB>(wp, cp, op, ioff, a.step, a.alpha, a.beta, q[f][3],
+                T* wp = (T*)(w0 + fo);
+                void* cp = KB ? (void*)(comp0 + fo / (long long)sizeof(T) * (KB / 8)) : nullptr;
+                fnew[f] = rk_update_pre<T, KB>(wp, cp, op, iob, a.step, a.alpha, a.beta, q[f][3],
                                                rhs[f], dt, wpre[f], cpre[f]);
             }
             if (a.step == 2) {
@@ -340,14 +358,14 @@ __global__ void __launch_bounds__(TX* TY) solve_tiled_kernel(SolveArgs<T> a, int
                 nv.u_up[f - 1] = upxy;
             }
             else if (f < 7) {
-                if (f == 4) nv.ax_dy = dy;
-                if (f == 5) nv.ay_dx = dx;
-                nv.a_lap[f - 4] = dxx + dyy;
+                if (f == 4) dl[dslot][0][tid] = dy;
+                if (f == 5) dl[dslot][1][tid] = dx;
+                dl[dslot][4 + (f - 4)][tid] = dxx + dyy;
             }
             else {
-                nv.s_dx = dx;
-                nv.s_dy = dy;
-                nv.s_lap = dxx + dyy;
+                dl[dslot][7][tid] = dx;
+                dl[dslot][8][tid] = dy;
+                dl[dslot][9][tid] = dxx + dyy;
             }
             // in-plane parts of grad(div v): x gets d_xx v_x, y gets d_yy v_y
             if (f == 1) ch[0][3] += dxx;
@@ -382,9 +400,12 @@ __global__ void __launch_bounds__(TX* TY) solve_tiled_kernel(SolveArgs<T> a, int
                 ch[c][i] = ch[c][i + 1];
             ch[c][6] = T(0);
         }
+        dl[dslot][2][tid] = az_dx_p;
+        dl[dslot][3][tid] = az_dy_p;
         dly[0] = dly[1];
         dly[1] = dly[2];
         dly[2] = dly[3];
+        dslot = dslot == 2 ? 0 : dslot + 1;
 
         // plane p + 1 has landed in the other buffer and every wave is done with this one
         __builtin_amdgcn_s_waitcnt(0);
@@ -397,313 +418,21 @@ __global__ void __launch_bounds__(TX* TY) solve_tiled_kernel(SolveArgs<T> a, int
     }
 }
 
-// ---------------------------------------------------------------------------------------------
-// Ring variant: the LDS holds a ring of five planes (the four planes c .. c+3 the output plane c
-// needs, plus the one being DMA'd). In-plane derivatives of the output plane and the forward
-// halves of the z stencils and of the xz / yz chains are read straight from the ring; only the
-// backward halves stay in registers (3 values per field, 3 partial sums per chain), so the
-// register state is small enough for two waves per SIMD with 512-thread workgroups.
-template <typename T, bool UPWD, bool ENTROPY, int KB, int TX, int TY>
-__global__ void __launch_bounds__(TX* TY) solve_ring_kernel(SolveArgs<T> a, int zchunk, int ntx,
-                                                            int nty, int nchunks)
-{
-    constexpr int LX = TX + 2 * R;
-    constexpr int LY = TY + 2 * R;
-    constexpr int LP = LX * LY;
-    constexpr int NT = TX * TY;
-    constexpr int NW = NT / 64;
-    constexpr int WPE = (int)sizeof(T) / 4;
-    constexpr int LPW = LP * WPE;
-    constexpr int NCHUNK = (LPW + 63) / 64;
-    constexpr int FPW = NCHUNK * 64;             // dwords per field-plane, padded to whole chunks
-    constexpr int FPE = FPW / WPE;               // elements per field-plane slot
-    constexpr int NR = (NCHUNK + NW - 1) / NW;
-    constexpr int NSLOT = 5;
-    static_assert(NT % 64 == 0, "workgroup must be whole waves");
-    static_assert(FPW % WPE == 0, "padding");
-    __shared__ T lds[NSLOT * NF * FPE];
-
-    const int nblocks = ntx * nty * nchunks;
-    const int bid = blockIdx.x;
-    int logical = bid;
-    if ((nblocks & 7) == 0)
-        logical = (bid & 7) * (nblocks >> 3) + (bid >> 3);
-    const int tile_x = logical % ntx;
-    const int rest = logical / ntx;
-    const int tile_y = rest % nty;
-    const int chunk = rest / nty;
-
-    const int tx = threadIdx.x, ty = threadIdx.y;
-    const int tid = tx + TX * ty;
-    const int wave = tid >> 6, lane = tid & 63;
-    const int x0 = tile_x * TX, y0 = tile_y * TY, z0 = chunk * zchunk;
-    const long long sy = a.mx, sz = a.sz;
-
-    // DMA slots of this lane; tail lanes of the last chunk re-read the plane's first dword into
-    // the padding, so every lane of a chunk is active and no per-lane predicate is needed.
-    unsigned g_off[NR];
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-        const int d = (wave + r * NW) * 64 + lane;
-        if (d < LPW) {
-            const int e = d / WPE, wd = d - e * WPE;
-            const int ly = e / LX, lx = e - ly * LX;
-            g_off[r] = (unsigned)(((x0 + lx) + (int)sy * (y0 + ly)) * (int)sizeof(T) + 4 * wd);
-        }
-        else {
-            g_off[r] = (unsigned)((x0 + (int)sy * y0) * (int)sizeof(T));
-        }
-    }
-    auto slot_of = [](int plane) { return ((plane % NSLOT) + NSLOT) % NSLOT; };
-    auto dma_plane = [&](int p) {
-        const long long base = sz * (long long)(p + R);
-        char* dst0 = (char*)&lds[slot_of(p) * NF * FPE];
-#pragma unroll
-        for (int f = 0; f < NF; ++f) {
-            const char* src = (const char*)(a.in[f] + base);
-            char* dst = dst0 + f * FPW * 4;
-#pragma unroll
-            for (int r = 0; r < NR; ++r) {
-                const int j = wave + r * NW;
-                if (j < NCHUNK)  // wave-uniform
-                    __builtin_amdgcn_global_load_lds((const void*)(src + g_off[r]),
-                                                     (lds_ptr_t)(dst + 256 * j), 4, 0, 0);
-            }
-        }
-    };
-
-    const Coeffs<T>& C = a.C;
-    T qb[NF][3];     // qb[f][k] = f(c - 3 + k): backward column values
-    T bk[NCH][3];    // bk[ch][k]: backward chain sums for plane c + k (k = 0: complete for c)
-#pragma unroll
-    for (int f = 0; f < NF; ++f)
-        qb[f][0] = qb[f][1] = qb[f][2] = T(0);
-#pragma unroll
-    for (int c = 0; c < NCH; ++c)
-        bk[c][0] = bk[c][1] = bk[c][2] = T(0);
-
-    const T dt = solve_dt(a);
-    double u2max = 0.0;
-    const int gx = x0 + tx, gy = y0 + ty;
-    const int ctr = (ty + R) * LX + (tx + R);
-    const unsigned ioff = (unsigned)((gx + R) + (int)sy * (gy + R));  // in-plane offset
-
-    for (int k = 0; k < 4; ++k)
-        dma_plane(z0 - R + k);
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-
-    for (int c = z0 - R; c < z0 + zchunk; ++c) {
-        if (c + 4 <= z0 + zchunk + R - 1)
-            dma_plane(c + 4);
-        const T* P0 = &lds[slot_of(c) * NF * FPE + ctr];
-        const T* P1 = &lds[slot_of(c + 1) * NF * FPE + ctr];
-        const T* P2 = &lds[slot_of(c + 2) * NF * FPE + ctr];
-        const T* P3 = &lds[slot_of(c + 3) * NF * FPE + ctr];
-        auto at = [&](const T* P, int f, int dx, int dy) { return P[f * FPE + dy * LX + dx]; };
-
-        const bool out_plane = c >= z0;
-        T nb[NCH][3];  // backward chain sums after this plane: nb[.][k] belongs to plane c + 1 + k
-        if (!out_plane) {
-            // warm-up plane below the chunk: feed the backward state only
-#pragma unroll
-            for (int v = 0; v < 2; ++v) {
-                const int fx = 1 + 3 * v, fy = 2 + 3 * v, fz = 3 + 3 * v;
-                T pxz[4], pyz[4], pz[4];
-#pragma unroll
-                for (int i = 1; i <= 3; ++i) {
-                    pxz[i] = C.mx[1][i] * (at(P0, fz, i, 0) - at(P0, fz, -i, 0));
-                    pyz[i] = C.mx[2][i] * (at(P0, fz, 0, i) - at(P0, fz, 0, -i));
-                    pz[i] = C.mx[1][i] * (at(P0, fx, i, 0) - at(P0, fx, -i, 0)) +
-                            C.mx[2][i] * (at(P0, fy, 0, i) - at(P0, fy, 0, -i));
-                }
-                nb[3 * v + 0][0] = bk[3 * v + 0][1] - pxz[1];
-                nb[3 * v + 0][1] = bk[3 * v + 0][2] - pxz[2];
-                nb[3 * v + 0][2] = -pxz[3];
-                nb[3 * v + 1][0] = bk[3 * v + 1][1] - pyz[1];
-                nb[3 * v + 1][1] = bk[3 * v + 1][2] - pyz[2];
-                nb[3 * v + 1][2] = -pyz[3];
-                nb[3 * v + 2][0] = bk[3 * v + 2][1] - pz[1];
-                nb[3 * v + 2][1] = bk[3 * v + 2][2] - pz[2];
-                nb[3 * v + 2][2] = -pz[3];
-            }
-        }
-        else {
-            // ---- (1) every field finished in one go: in-plane (ring slot of plane c) and z
-            // (column queue backwards, ring forwards) parts of gradient, Laplacian and upwind.
-            CellDerivs<T> d;
-            T val[NF];
-            T gdv[2][3];
-            const T aux = ac_abs(at(P0, 1, 0, 0)), auy = ac_abs(at(P0, 2, 0, 0)),
-                    auz = ac_abs(at(P0, 3, 0, 0));
-#pragma unroll
-            for (int f = 0; f < NF; ++f) {
-                const T v0 = at(P0, f, 0, 0);
-                val[f] = v0;
-                T Dx[4], Dy[4], Sx[4], Sy[4], Dz[4], Sz[4];
-#pragma unroll
-                for (int i = 1; i <= 3; ++i) {
-                    const T xp = at(P0, f, i, 0), xm = at(P0, f, -i, 0);
-                    const T yp = at(P0, f, 0, i), ym = at(P0, f, 0, -i);
-                    const T* Pi = i == 1 ? P1 : (i == 2 ? P2 : P3);
-                    const T zp = at(Pi, f, 0, 0), zm = qb[f][3 - i];
-                    Dx[i] = xp - xm;
-                    Sx[i] = xp + xm;
-                    Dy[i] = yp - ym;
-                    Sy[i] = yp + ym;
-                    Dz[i] = zp - zm;
-                    Sz[i] = zp + zm;
-                }
-                const T gx_ = C.d1[0][1] * Dx[1] + C.d1[0][2] * Dx[2] + C.d1[0][3] * Dx[3];
-                const T gy_ = C.d1[1][1] * Dy[1] + C.d1[1][2] * Dy[2] + C.d1[1][3] * Dy[3];
-                const T gz_ = C.d1[2][1] * Dz[1] + C.d1[2][2] * Dz[2] + C.d1[2][3] * Dz[3];
-                const T dxx = C.d2[0][1] * Sx[1] + C.d2[0][2] * Sx[2] + C.d2[0][3] * Sx[3];
-                const T dyy = C.d2[1][1] * Sy[1] + C.d2[1][2] * Sy[2] + C.d2[1][3] * Sy[3];
-                const T dzz = C.d2[2][1] * Sz[1] + C.d2[2][2] * Sz[2] + C.d2[2][3] * Sz[3];
-                const T c0x = C.d2[0][0] * v0, c0y = C.d2[1][0] * v0, c0z = C.d2[2][0] * v0;
-                const T lap = (dxx + dyy + dzz) + (c0x + c0y + c0z);
-                T upw = T(0);
-                if (UPWD && f < 4) {
-                    const T ux_ = C.up[0][0] * v0 + C.up[0][1] * Sx[1] + C.up[0][2] * Sx[2] + C.up[0][3] * Sx[3];
-                    const T uy_ = C.up[1][0] * v0 + C.up[1][1] * Sy[1] + C.up[1][2] * Sy[2] + C.up[1][3] * Sy[3];
-                    const T uz_ = C.up[2][0] * v0 + C.up[2][1] * Sz[1] + C.up[2][2] * Sz[2] + C.up[2][3] * Sz[3];
-                    upw = aux * ux_ + auy * uy_ + auz * uz_;
-                }
-                if (f == 0) {
-                    d.lr = v0;
-                    d.glr[0] = gx_;
-                    d.glr[1] = gy_;
-                    d.glr[2] = gz_;
-                    d.laplr = lap;
-                    d.upl = upw;
-                }
-                else if (f < 4) {
-                    d.u[f - 1] = v0;
-                    d.gu[f - 1][0] = gx_;
-                    d.gu[f - 1][1] = gy_;
-                    d.gu[f - 1][2] = gz_;
-                    d.lapu[f - 1] = lap;
-                    d.upu[f - 1] = upw;
-                }
-                else if (f < 7) {
-                    d.ga[f - 4][0] = gx_;
-                    d.ga[f - 4][1] = gy_;
-                    d.ga[f - 4][2] = gz_;
-                    d.lapa[f - 4] = lap;
-                }
-                else {
-                    d.s = v0;
-                    d.gs[0] = gx_;
-                    d.gs[1] = gy_;
-                    d.gs[2] = gz_;
-                    d.laps = lap;
-                }
-                // diagonal parts of grad(div v) and the backward chain pushes of plane c
-                if (f == 1 || f == 4) {
-                    const int v = f == 1 ? 0 : 1;
-                    gdv[v][0] = dxx + c0x;
-                    nb[3 * v + 2][0] = bk[3 * v + 2][1] - C.mx[1][1] * Dx[1];
-                    nb[3 * v + 2][1] = bk[3 * v + 2][2] - C.mx[1][2] * Dx[2];
-                    nb[3 * v + 2][2] = -C.mx[1][3] * Dx[3];
-                }
-                if (f == 2 || f == 5) {
-                    const int v = f == 2 ? 0 : 1;
-                    gdv[v][1] = dyy + c0y;
-                    nb[3 * v + 2][0] -= C.mx[2][1] * Dy[1];
-                    nb[3 * v + 2][1] -= C.mx[2][2] * Dy[2];
-                    nb[3 * v + 2][2] -= C.mx[2][3] * Dy[3];
-                }
-                if (f == 3 || f == 6) {
-                    const int v = f == 3 ? 0 : 1;
-                    gdv[v][2] = dzz + c0z;
-                    nb[3 * v + 0][0] = bk[3 * v + 0][1] - C.mx[1][1] * Dx[1];
-                    nb[3 * v + 0][1] = bk[3 * v + 0][2] - C.mx[1][2] * Dx[2];
-                    nb[3 * v + 0][2] = -C.mx[1][3] * Dx[3];
-                    nb[3 * v + 1][0] = bk[3 * v + 1][1] - C.mx[2][1] * Dy[1];
-                    nb[3 * v + 1][1] = bk[3 * v + 1][2] - C.mx[2][2] * Dy[2];
-                    nb[3 * v + 1][2] = -C.mx[2][3] * Dy[3];
-                }
-                __builtin_amdgcn_sched_barrier(0);
-            }
-            // ---- (2) off-diagonal parts of grad(div v): xy in-plane, xz / yz from the chains
-            // (forward halves read from the ring, backward halves from bk)
-#pragma unroll
-            for (int v = 0; v < 2; ++v) {
-                const int fx = 1 + 3 * v, fy = 2 + 3 * v, fz = 3 + 3 * v;
-                T mxx = T(0), myy = T(0);
-                T ex = bk[3 * v + 0][0], ey = bk[3 * v + 1][0], ez = bk[3 * v + 2][0];
-#pragma unroll
-                for (int i = 1; i <= 3; ++i) {
-                    mxx += C.mx[0][i] * ((at(P0, fx, i, i) + at(P0, fx, -i, -i)) - (at(P0, fx, i, -i) + at(P0, fx, -i, i)));
-                    myy += C.mx[0][i] * ((at(P0, fy, i, i) + at(P0, fy, -i, -i)) - (at(P0, fy, i, -i) + at(P0, fy, -i, i)));
-                    const T* Pi = i == 1 ? P1 : (i == 2 ? P2 : P3);
-                    ex += C.mx[1][i] * (at(Pi, fz, i, 0) - at(Pi, fz, -i, 0));
-                    ey += C.mx[2][i] * (at(Pi, fz, 0, i) - at(Pi, fz, 0, -i));
-                    ez += C.mx[1][i] * (at(Pi, fx, i, 0) - at(Pi, fx, -i, 0)) +
-                          C.mx[2][i] * (at(Pi, fy, 0, i) - at(Pi, fy, 0, -i));
-                }
-                T* gd = v == 0 ? d.gdu : d.gda;
-                gd[0] = gdv[v][0] + myy + ex;
-                gd[1] = gdv[v][1] + mxx + ey;
-                gd[2] = gdv[v][2] + ez;
-                __builtin_amdgcn_sched_barrier(0);
-            }
-
-            // ---- (3) right-hand side and the RK3 update of the cell
-            T rhs[NF];
-            mhd_rhs<T, UPWD, ENTROPY>(d, a.P, rhs);
-            const long long poff = sz * (long long)(c + R);
-            T fnew[NF];
-#pragma unroll
-            for (int f = 0; f < NF; ++f) {
-                T* op = a.out[f] + poff;
-                if (f == 7 && !ENTROPY) {
-                    fnew[f] = val[f];
-                    op[ioff] = val[f];
-                    continue;
-                }
-                T* wp = a.w[f] + poff;
-                void* cp = KB ? (void*)((char*)a.comp[f] + poff * (KB / 8)) : nullptr;
-                fnew[f] = rk_update_plane<T, KB>(wp, cp, op, ioff, a.step, a.alpha, a.beta, val[f],
-                                                 rhs[f], dt);
-            }
-            if (a.step == 2) {
-                const double u2 = (double)fnew[1] * (double)fnew[1] + (double)fnew[2] * (double)fnew[2] +
-                                  (double)fnew[3] * (double)fnew[3];
-                u2max = fmax(u2max, u2);
-            }
-        }
-        // ---- (6) slide the window by one plane
-#pragma unroll
-        for (int ch = 0; ch < NCH; ++ch)
-#pragma unroll
-            for (int k = 0; k < 3; ++k)
-                bk[ch][k] = nb[ch][k];
-#pragma unroll
-        for (int f = 0; f < NF; ++f) {
-            qb[f][0] = qb[f][1];
-            qb[f][1] = qb[f][2];
-            qb[f][2] = at(P0, f, 0, 0);
-        }
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
-    }
-    if (a.step == 2 && a.umax2_next != nullptr) {
-        wave_max_to_global(u2max, a.umax2_next);
-        if (a.time_acc != nullptr && blockIdx.x == 0 && tid == 0)
-            *a.time_acc += (double)dt;
-    }
-}
-
-// Launch shapes (TX, TY, ZC). (32, 16) selects the five-plane ring kernel (FP32 only: its ring
-// of 5 x 8 x 38 x 22 values fills 140 KiB of the 160 KiB LDS); the others the delay-line kernel.
+// Launch shapes (TX, TY, ZC): 64 x 4 and 64 x 2 run the delay-line kernel above; 32 x 8,
+// 16 x 16 and 32 x 16 the ring kernel of solve_ring.hip (32 x 16 in FP32 only: its FP64 ring
+// would exceed the 160 KiB LDS).
 static const TiledShape kShapes[] = {
-    {32, 16, 32}, {32, 16, 64}, {32, 16, 128}, {32, 16, 256},
     {64, 4, 32}, {64, 4, 64}, {64, 4, 128}, {64, 4, 256},
-    {32, 8, 64}, {32, 8, 128}, {64, 2, 64}, {64, 2, 128},
+    {64, 2, 64}, {64, 2, 128},
+    {32, 8, 32}, {32, 8, 64}, {32, 8, 128}, {32, 8, 256},
+    {16, 16, 32}, {16, 16, 64}, {16, 16, 128}, {16, 16, 256},
+    {32, 16, 64}, {32, 16, 128}, {32, 16, 256},
 };
 
-static bool is_ring(TiledShape s) { return s.tx == 32 && s.ty == 16; }
+static bool is_ring(TiledShape s)
+{
+    return (s.tx == 32 && s.ty == 8) || (s.tx == 16 && s.ty == 16) || (s.tx == 32 && s.ty == 16);
+}
 
 int tiled_num_shapes() { return (int)(sizeof(kShapes) / sizeof(kShapes[0])); }
 TiledShape tiled_shape(int i) { return kShapes[i]; }
@@ -713,7 +442,7 @@ bool tiled_shape_fits(TiledShape s, int nx, int ny, int nz)
     bool known = false;
     for (const TiledShape& k : kShapes)
         known = known || (k.tx == s.tx && k.ty == s.ty);
-    if (is_ring(s) && sizeof(AcRealT) != 4)
+    if (s.tx == 32 && s.ty == 16 && sizeof(AcRealT) != 4)
         return false;
     return known && s.zchunk > 0 && nx % s.tx == 0 && ny % s.ty == 0 && nz % s.zchunk == 0;
 }
@@ -728,29 +457,26 @@ static void launch_shape(const SolveArgs<T>& a, int zchunk, hipStream_t s)
                        zchunk, ntx, nty, nch);
 }
 
-template <typename T, bool UPWD, bool ENTROPY, int KB, int TX, int TY>
-static void launch_ring(const SolveArgs<T>& a, int zchunk, hipStream_t s)
-{
-    const int ntx = a.nx / TX, nty = a.ny / TY, nch = a.nz / zchunk;
-    const dim3 tpb(TX, TY, 1);
-    const dim3 bpg((unsigned)(ntx * nty * nch), 1, 1);
-    hipLaunchKernelGGL((solve_ring_kernel<T, UPWD, ENTROPY, KB, TX, TY>), bpg, tpb, 0, s, a,
-                       zchunk, ntx, nty, nch);
-}
-
 template <typename T, bool UPWD, bool ENTROPY, int KB>
 static bool launch_kb(const SolveArgs<T>& a, TiledShape sh, hipStream_t s)
 {
-    if (sh.tx == 32 && sh.ty == 16) {
-        if constexpr (sizeof(T) == 4)
-            launch_ring<T, UPWD, ENTROPY, KB, 32, 16>(a, sh.zchunk, s);
-        else
-            return false;
-    }
-    else if (sh.tx == 64 && sh.ty == 4) launch_shape<T, UPWD, ENTROPY, KB, 64, 4>(a, sh.zchunk, s);
-    else if (sh.tx == 32 && sh.ty == 8) launch_shape<T, UPWD, ENTROPY, KB, 32, 8>(a, sh.zchunk, s);
+    if (sh.tx == 64 && sh.ty == 4) launch_shape<T, UPWD, ENTROPY, KB, 64, 4>(a, sh.zchunk, s);
     else if (sh.tx == 64 && sh.ty == 2) launch_shape<T, UPWD, ENTROPY, KB, 64, 2>(a, sh.zchunk, s);
     else return false;
+    return true;
+}
+
+// The tiled kernels address field f of every role as slab + f * fstride.
+template <typename T>
+static bool uniform_slabs(const SolveArgs<T>& a, int kb)
+{
+    for (int q = 0; q < NF; ++q) {
+        const long long o = q * a.fstride;
+        if (a.in[q] != a.in[0] + o || a.out[q] != a.out[0] + o || a.w[q] != a.w[0] + o)
+            return false;
+        if (kb > 0 && (const char*)a.comp[q] != (const char*)a.comp[0] + o * (kb / 8))
+            return false;
+    }
     return true;
 }
 
@@ -758,8 +484,10 @@ template <typename T>
 bool launch_solve_tiled(const SolveArgs<T>& a, bool upwd, bool entropy, int kb, TiledShape sh,
                         hipStream_t s)
 {
-    if (!tiled_shape_fits(sh, a.nx, a.ny, a.nz))
+    if (!tiled_shape_fits(sh, a.nx, a.ny, a.nz) || !uniform_slabs(a, kb))
         return false;
+    if (is_ring(sh))
+        return launch_solve_ring<T>(a, upwd, entropy, kb, sh, s);
 #define AC_TILED_KB(U, E)                                                                   \
     do {                                                                                    \
         if (kb == 0) return launch_kb<T, U, E, 0>(a, sh, s);                                \

@@ -59,16 +59,19 @@ def lib(request, require_gpu):
     return ac.load(request.param)
 
 
-RING = (32, 16)  # five-plane ring kernel, FP32 only
+# Tiles 64 x 4 and 64 x 2 run the delay-line kernel; 32 x 8, 16 x 16 and 32 x 16 the four-plane
+# ring kernel, whose 32 x 16 ring does not fit the LDS in FP64.
+RING_FP32_ONLY = (32, 16)
 
 
 def skip_ring_fp64(lib, shape):
-    if tuple(shape[:2]) == RING and lib.precision == "fp64":
-        pytest.skip("the ring kernel exists for FP32 only (its LDS ring does not fit FP64)")
+    if tuple(shape[:2]) == RING_FP32_ONLY and lib.precision == "fp64":
+        pytest.skip("the 32 x 16 ring exists for FP32 only (its FP64 ring exceeds the LDS)")
 
 
-@pytest.mark.parametrize("n,shape", [((64, 32, 32), (64, 4, 32)), ((32, 32, 64), (32, 8, 64)),
-                                     ((64, 16, 64), (64, 2, 64)), ((32, 32, 32), (32, 16, 32)),
+@pytest.mark.parametrize("n,shape", [((64, 32, 32), (64, 4, 32)), ((64, 16, 64), (64, 2, 64)),
+                                     ((32, 32, 64), (32, 8, 64)), ((32, 32, 32), (32, 8, 32)),
+                                     ((32, 48, 64), (16, 16, 32)), ((32, 32, 32), (32, 16, 32)),
                                      ((64, 48, 64), (32, 16, 64))])
 @pytest.mark.parametrize("upwd,entropy", [(True, True), (False, False)])
 def test_tiled_vs_oracle(lib, n, shape, upwd, entropy):
@@ -89,7 +92,8 @@ def test_tiled_vs_oracle(lib, n, shape, upwd, entropy):
         assert e["maxrel"] < tol, (q, e)
 
 
-@pytest.mark.parametrize("shape", [(64, 4, 64), (64, 4, 128), (32, 8, 128), (32, 16, 64)])
+@pytest.mark.parametrize("shape", [(64, 4, 64), (64, 4, 128), (32, 8, 128), (16, 16, 64),
+                                   (32, 16, 64)])
 def test_tiled_vs_direct_128(lib, shape):
     skip_ring_fp64(lib, shape)
     say(f"case tiled vs direct 128^3 {lib.precision} shape={shape}")
@@ -109,7 +113,7 @@ def test_tiled_vs_direct_128(lib, shape):
 
 
 @pytest.mark.parametrize("kb,shape", [(8, (64, 4, 32)), (16, (64, 4, 32)), (32, (64, 4, 32)),
-                                      (8, (32, 16, 32)), (16, (32, 16, 32))])
+                                      (8, (32, 8, 32)), (16, (16, 16, 32)), (32, (32, 16, 32))])
 def test_tiled_kahan_closer_to_fp64(kb, shape, require_gpu):
     """Kahan-compensated FP32 stays closer to the FP64 solution than plain FP32."""
     say(f"case tiled kahan {kb} shape={shape}")
