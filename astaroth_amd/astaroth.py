@@ -87,7 +87,8 @@ EXPORTED = [
     "acHostMeshInitFourier", "acGetRealSize", "acGetMeshInfoSize", "acGetMeshSize",
     "acGetBuildInfo",
     "acAdjointWorkspaceBytes", "acDeviceIntegrateSubstepTangent", "acDeviceIntegrateSubstepAdjoint",
-    "acDeviceLoadMeshFromDevicePtr", "acDeviceStoreMeshToDevicePtr",
+    "acDeviceLoadMeshFromDevicePtr", "acDeviceStoreMeshToDevicePtr", "acDeviceInitFourier",
+    "acHostAppendTimeseries", "acGetVertexBufferName",
 ]
 
 
@@ -104,7 +105,10 @@ class AcDiagnostics(ctypes.Structure):
                 ("max", ctypes.c_double * NUM_VTXBUF_HANDLES),
                 ("rms", ctypes.c_double * NUM_VTXBUF_HANDLES),
                 ("uu_rms", ctypes.c_double), ("uu_max", ctypes.c_double),
-                ("bb_rms", ctypes.c_double), ("bb_max", ctypes.c_double)]
+                ("bb_rms", ctypes.c_double), ("bb_max", ctypes.c_double),
+                ("uu_min", ctypes.c_double), ("bb_min", ctypes.c_double),
+                ("va_min", ctypes.c_double), ("va_rms", ctypes.c_double),
+                ("va_max", ctypes.c_double)]
 
 
 class AcReduction(ctypes.Structure):
@@ -276,7 +280,11 @@ class AstarothLib:
                                                 ctypes.c_void_p, ctypes.c_void_p],
             "acDeviceLoadMeshFromDevicePtr": [dev, i, ctypes.c_void_p],
             "acDeviceStoreMeshToDevicePtr": [dev, i, ctypes.c_void_p],
+            "acDeviceInitFourier": [dev, i, ctypes.c_ulonglong, i, i, ctypes.c_double,
+                                    ctypes.c_double],
         }
+        sig["acHostAppendTimeseries"] = [ctypes.c_char_p, i, ctypes.c_double, ctypes.c_double,
+                                         ctypes.POINTER(AcDiagnostics), ctypes.POINTER(ctypes.c_int)]
         for name, args in sig.items():
             if not hasattr(L, name):
                 continue
@@ -284,6 +292,8 @@ class AstarothLib:
             fn.argtypes = args
             fn.restype = ctypes.c_int
             setattr(self, name, fn)
+        L.acGetVertexBufferName.argtypes = [i]
+        L.acGetVertexBufferName.restype = ctypes.c_char_p
         for name in ("acDeviceGetConfig", "acDeviceGetLocalConfig"):
             fn = getattr(L, name)
             fn.argtypes = [dev]
@@ -406,6 +416,11 @@ class Device:
 
     def sync(self, stream=STREAM_DEFAULT):
         check(self.lib.acDeviceSynchronizeStream(self.handle, stream), "sync")
+
+    def init_fourier(self, seed, nmodes, kmax, ampl, lnrho_perturb=0.0, stream=STREAM_DEFAULT):
+        """acDeviceInitFourier: the seeded Fourier initial condition generated on the device."""
+        check(self.lib.acDeviceInitFourier(self.handle, stream, seed, nmodes, kmax, ampl,
+                                           lnrho_perturb), "acDeviceInitFourier")
 
     def reduce_scal(self, name, field, post=True):
         out = self.lib.c_real()

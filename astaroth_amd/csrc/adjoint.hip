@@ -4,11 +4,12 @@
 // Tangent:  dw' = alpha_s dw + dt (J(f) df + dR/dnu dnu + dR/deta deta),  df' = df + beta_s dw'.
 // Adjoint:  mu = lw' + beta_s lf',  lw = alpha_s mu,  lf = lf' + dt J(f)^T mu,
 //           lnu += dt <mu, dR/dnu>,  leta += dt <mu, dR/deta>.
-// J^T mu is computed in three launches: (A) per cell, the adjoint of every derivative quantity
+// J^T mu is computed in three stages: (A) per cell, the adjoint of every derivative quantity
 // the right-hand side reads (55 fields, "derivative adjoints"); (B) their periodic halo fill,
-// which is the transpose of the forward halo copy; (C) per cell, the transposed stencils
-// gathered from the derivative adjoints (first derivatives are antisymmetric, so their
-// transpose is the negative; second, upwind and mixed ones are symmetric).
+// which is the transpose of the forward halo copy; (C) per output field, the transposed
+// stencils applied to the derivative adjoints by a z-marching kernel (first derivatives are
+// antisymmetric, so their transpose is the negative; second, upwind and mixed ones are
+// symmetric).
 #include <cstdio>
 #include <cstring>
 
@@ -301,88 +302,176 @@ __global__ void __launch_bounds__(256) adjoint_param_stage2(const double* ranges
     }
 }
 
-// ---------------------------------------------------------------- adjoint (C): transposed stencils
-template <typename T>
-__device__ __forceinline__ T scalar_transpose(const T* g, long long n, int base, long long c,
-                                              const long long* st, const Coeffs<T>& C, bool has_val,
-                                              bool has_up)
-{
-    // val - sum_a D1_a(g_1a) + sum_a D2_a(g_lap) + sum_a U_a(g_up_a)
-    T r = has_val ? g[(long long)base * n + c] : T(0);
-    const T* gl = g + (long long)(base + 4) * n;
-#pragma unroll
-    for (int ax = 0; ax < 3; ++ax) {
-        r -= d1(g + (long long)(base + 1 + ax) * n, c, st[ax], C.d1[ax]);
-        r += d2(gl, c, st[ax], C.d2[ax]);
-        if (has_up)
-            r += d2(g + (long long)(base + 5 + ax) * n, c, st[ax], C.up[ax]);
-    }
-    return r;
-}
-
-template <typename T, bool UPWD, bool ENTROPY>
-__global__ void __launch_bounds__(256) adjoint_transpose_kernel(AdjArgs<T> A)
+// ---------------------------------------------------------------- adjoint (C), z-marching form
+// The same transposed stencils, organised so that every derivative-adjoint value is read from
+// memory once per use pattern: a thread owns an (x, y) column and walks a chunk of planes. Each
+// output field q has a chain acc[q][k] of partial sums for planes p - 3 + k (k = 0..6), where p
+// is the plane being read. Reading plane p pushes
+//   * its pointwise and in-plane (x / y) contributions into acc[q][3] (plane p itself);
+//   * its z-stencil contributions into acc[q][3 -+ i]: a value a(p) under the transposed first
+//     derivative lands on plane p - i with -d1_i and on p + i with +d1_i; under second
+//     derivative, upwind and D2z of grad(div) with the symmetric weights;
+//   * the xz / yz mixed parts: D_i G(p) = G(x+i, p) - G(x-i, p) lands on p - i with +c_i and on
+//     p + i with -c_i.
+// Plane p - 3 is then complete (acc[q][0]) and its adjoint update is written.
+template <typename T, bool UPWD, int Q>
+__global__ void __launch_bounds__(256) adjoint_transpose_z_kernel(AdjArgs<T> A, int zchunk)
 {
     const SolveArgs<T>& a = A.s;
-    bool active;
-    long long c;
-    cell_index(a, active, c);
-    if (!active)
+    const int i = threadIdx.x + blockIdx.x * blockDim.x;
+    const int j = threadIdx.y + blockIdx.y * blockDim.y;
+    const int z0 = blockIdx.z * zchunk;
+    if (i >= a.nx || j >= a.ny)
         return;
-    const long long n = A.count;
-    const long long st[3] = {1, (long long)a.mx, a.sz};
-    const T* g = A.g;
+    const int sy = a.mx;
+    const long long sz = a.sz;
     const Coeffs<T>& C = a.C;
-    T tr[NF];
-    tr[0] = scalar_transpose(g, n, G_LR, c, st, C, true, UPWD);
+    const unsigned colb0 = (unsigned)(((i + R) + sy * (j + R)) * (int)sizeof(T));
+    T acc[7];
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
-        tr[1 + i] = scalar_transpose(g, n, G_U + 8 * i, c, st, C, true, UPWD);
-    // A fields: D1 (3) and lap only
+    for (int k = 0; k < 7; ++k)
+        acc[k] = T(0);
+
+    const int np = zchunk + 2 * R;
+    for (int it = 0; it < np; ++it) {
+        const int p = z0 - R + it;  // interior plane index being read (array plane p + R)
+        // uniform bases recomputed per plane from an opaque field stride; 32-bit lane offsets
+        const long long nb = opaque_s64(A.count * (long long)sizeof(T));
+        const unsigned colb = opaque(colb0);
+        const char* const pb = (const char*)A.g + sz * (long long)(p + R) * (long long)sizeof(T);
+        // value of derivative-adjoint field k at offset (dx, dy) from this column in plane p
+        auto ld = [&](int k, int dx, int dy) -> T {
+            return *(const T*)(pb + k * nb + colb + (unsigned)((dx + dy * sy) * (int)sizeof(T)));
+        };
+        auto d1x = [&](int k) {
+            return C.d1[0][1] * (ld(k, 1, 0) - ld(k, -1, 0)) + C.d1[0][2] * (ld(k, 2, 0) - ld(k, -2, 0)) +
+                   C.d1[0][3] * (ld(k, 3, 0) - ld(k, -3, 0));
+        };
+        auto d1y = [&](int k) {
+            return C.d1[1][1] * (ld(k, 0, 1) - ld(k, 0, -1)) + C.d1[1][2] * (ld(k, 0, 2) - ld(k, 0, -2)) +
+                   C.d1[1][3] * (ld(k, 0, 3) - ld(k, 0, -3));
+        };
+        // symmetric stencil along x (ax = 0) or y (ax = 1), centre weight included
+        auto sym = [&](int k, int ax, const T* w) {
+            const int ex = ax == 0, ey = ax == 1;
+            return w[0] * ld(k, 0, 0) + w[1] * (ld(k, ex, ey) + ld(k, -ex, -ey)) +
+                   w[2] * (ld(k, 2 * ex, 2 * ey) + ld(k, -2 * ex, -2 * ey)) +
+                   w[3] * (ld(k, 3 * ex, 3 * ey) + ld(k, -3 * ex, -3 * ey));
+        };
+        auto mixed_xy = [&](int k) {
+            T r = T(0);
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        T r = T(0);
-        const T* gl = g + (long long)(G_A + 4 * i + 3) * n;
+            for (int o = 1; o <= 3; ++o)
+                r += C.mx[0][o] * ((ld(k, o, o) + ld(k, -o, -o)) - (ld(k, o, -o) + ld(k, -o, o)));
+            return r;
+        };
+        // transposed first derivative along z of a column value: -d1_o on p - o, +d1_o on p + o
+        auto push_anti = [&](T v) {
 #pragma unroll
-        for (int ax = 0; ax < 3; ++ax) {
-            r -= d1(g + (long long)(G_A + 4 * i + ax) * n, c, st[ax], C.d1[ax]);
-            r += d2(gl, c, st[ax], C.d2[ax]);
+            for (int o = 1; o <= 3; ++o) {
+                acc[3 - o] -= C.d1[2][o] * v;
+                acc[3 + o] += C.d1[2][o] * v;
+            }
+        };
+        auto push_sym = [&](T v, const T* w) {
+            acc[3] += w[0] * v;
+#pragma unroll
+            for (int o = 1; o <= 3; ++o) {
+                acc[3 - o] += w[o] * v;
+                acc[3 + o] += w[o] * v;
+            }
+        };
+        // xz (ax = 0) or yz (ax = 1) mixed part of field k: D_o G(p) on p - o with +c_o, on
+        // p + o with -c_o
+        auto push_mixed = [&](int k, int ax) {
+            const int ex = ax == 0, ey = ax == 1;
+            const T* w = C.mx[ax == 0 ? 1 : 2];
+#pragma unroll
+            for (int o = 1; o <= 3; ++o) {
+                const T D = w[o] * (ld(k, o * ex, o * ey) - ld(k, -o * ex, -o * ey));
+                acc[3 - o] += D;
+                acc[3 + o] -= D;
+            }
+        };
+
+        if (Q < 4) {
+            // lnrho and u: val, D1 x y z, lap, up x y z
+            const int b = Q == 0 ? G_LR : G_U + 8 * (Q - 1);
+            T r = ld(b + 0, 0, 0) - (d1x(b + 1) + d1y(b + 2));
+            r += sym(b + 4, 0, C.d2[0]) + sym(b + 4, 1, C.d2[1]);
+            if (UPWD)
+                r += sym(b + 5, 0, C.up[0]) + sym(b + 6, 1, C.up[1]);
+            acc[3] += r;
+            push_anti(ld(b + 3, 0, 0));
+            push_sym(ld(b + 4, 0, 0), C.d2[2]);
+            if (UPWD)
+                push_sym(ld(b + 7, 0, 0), C.up[2]);
         }
-        tr[4 + i] = r;
-    }
-    // ss: val, D1, lap (base layout differs: val, D1 x y z, lap)
-    {
-        T r = g[(long long)G_S * n + c];
-        const T* gl = g + (long long)(G_S + 4) * n;
-#pragma unroll
-        for (int ax = 0; ax < 3; ++ax) {
-            r -= d1(g + (long long)(G_S + 1 + ax) * n, c, st[ax], C.d1[ax]);
-            r += d2(gl, c, st[ax], C.d2[ax]);
+        else if (Q < 7) {
+            // A: D1 x y z, lap
+            const int b = G_A + 4 * (Q - 4);
+            acc[3] += sym(b + 3, 0, C.d2[0]) + sym(b + 3, 1, C.d2[1]) - (d1x(b + 0) + d1y(b + 1));
+            push_anti(ld(b + 2, 0, 0));
+            push_sym(ld(b + 3, 0, 0), C.d2[2]);
         }
-        tr[7] = r;
-    }
-    // grad(div v): forward gd_i = sum_j S_ij v_j with S_ii = D2_i, S_ij = M_ij (symmetric)
-#pragma unroll
-    for (int v = 0; v < 2; ++v) {
-        const int gb = v == 0 ? G_GDU : G_GDA;
-        const T* G0 = g + (long long)(gb + 0) * n;
-        const T* G1 = g + (long long)(gb + 1) * n;
-        const T* G2 = g + (long long)(gb + 2) * n;
-        tr[1 + 3 * v] += d2(G0, c, st[0], C.d2[0]) + dm(G1, c, st[0], st[1], C.mx[0]) + dm(G2, c, st[0], st[2], C.mx[1]);
-        tr[2 + 3 * v] += dm(G0, c, st[0], st[1], C.mx[0]) + d2(G1, c, st[1], C.d2[1]) + dm(G2, c, st[1], st[2], C.mx[2]);
-        tr[3 + 3 * v] += dm(G0, c, st[0], st[2], C.mx[1]) + dm(G1, c, st[1], st[2], C.mx[2]) + d2(G2, c, st[2], C.d2[2]);
-    }
-#pragma unroll
-    for (int q = 0; q < NF; ++q) {
-        const long long o = (long long)q * n + c;
-        if (q == 7 && !ENTROPY) {
-            A.lw[o] = T(0);  // f'_s = f_s: lf passes through unchanged
-            continue;
+        else {
+            // ss: val, D1 x y z, lap
+            const int b = G_S;
+            T r = ld(b, 0, 0) - (d1x(b + 1) + d1y(b + 2));
+            r += sym(b + 4, 0, C.d2[0]) + sym(b + 4, 1, C.d2[1]);
+            acc[3] += r;
+            push_anti(ld(b + 3, 0, 0));
+            push_sym(ld(b + 4, 0, 0), C.d2[2]);
         }
-        const T mu = A.lw[o] + a.beta * A.lf[o];
-        A.lf[o] = A.lf[o] + a.dt * tr[q];
-        A.lw[o] = a.alpha * mu;
+        // grad(div v) parts of the vector components (u: Q = 1..3 from G_GDU, A: 4..6 from G_GDA)
+        if (Q >= 1 && Q <= 6) {
+            const int gb = Q <= 3 ? G_GDU : G_GDA;
+            const int comp = (Q - 1) % 3;
+            if (comp == 0) {
+                acc[3] += sym(gb + 0, 0, C.d2[0]) + mixed_xy(gb + 1);
+                push_mixed(gb + 2, 0);  // d_xz of G2
+            }
+            else if (comp == 1) {
+                acc[3] += mixed_xy(gb + 0) + sym(gb + 1, 1, C.d2[1]);
+                push_mixed(gb + 2, 1);  // d_yz of G2
+            }
+            else {
+                push_mixed(gb + 0, 0);  // d_xz of G0
+                push_mixed(gb + 1, 1);  // d_yz of G1
+                push_sym(ld(gb + 2, 0, 0), C.d2[2]);
+            }
+        }
+
+        // ---- plane p - 3 is complete: its adjoint update
+        if (it >= 2 * R) {
+            const long long ob = sz * (long long)p * (long long)sizeof(T);  // array plane p - 3 + R
+            T* lfp = (T*)((char*)A.lf + Q * nb + ob + colb);
+            T* lwp = (T*)((char*)A.lw + Q * nb + ob + colb);
+            const T lfo = *lfp;
+            const T mu = *lwp + a.beta * lfo;
+            *lfp = lfo + a.dt * acc[0];
+            *lwp = a.alpha * mu;
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            acc[k] = acc[k + 1];
+        acc[6] = T(0);
     }
+}
+
+// Without the entropy equation ss is not evolved (f'_7 = f_7): its adjoint passes through and
+// lambda_w of ss is zero.
+template <typename T>
+__global__ void __launch_bounds__(256) adjoint_ss_passive_kernel(AdjArgs<T> A)
+{
+    const SolveArgs<T>& a = A.s;
+    const int i = threadIdx.x + blockIdx.x * blockDim.x;
+    const int j = threadIdx.y + blockIdx.y * blockDim.y;
+    const int k = blockIdx.z;
+    if (i >= a.nx || j >= a.ny || k >= a.nz)
+        return;
+    const long long c = (long long)(i + R) + (long long)a.mx * (j + R) + a.sz * (k + R);
+    A.lw[7 * A.count + c] = T(0);
 }
 
 // ---------------------------------------------------------------- launchers
@@ -434,7 +523,31 @@ void launch_adjoint(const AdjArgs<T>& A, bool upwd, bool entropy, double* param_
             ptrs[q] = A.g + (long long)(k0 + q) * A.count;
         launch_periodic_fill<T>(ptrs, cnt, A.s.nx, A.s.ny, A.s.nz, s);
     }
-    AC_ADJ(adjoint_transpose_kernel);
+    // (C) transposed stencils, z-marching over chunks of planes
+    int zc = A.s.nz;
+    for (int cand : {32, 16, 8})
+        if (A.s.nz % cand == 0 && A.s.nz > cand) {
+            zc = cand;
+            break;
+        }
+    const dim3 bpz(bpg.x, bpg.y, (unsigned)(A.s.nz / zc));
+#define AC_ADJ_Q(Q)                                                                                \
+    do {                                                                                           \
+        if (upwd) hipLaunchKernelGGL((adjoint_transpose_z_kernel<T, true, Q>), bpz, tpb, 0, s, A, zc); \
+        else hipLaunchKernelGGL((adjoint_transpose_z_kernel<T, false, Q>), bpz, tpb, 0, s, A, zc);    \
+    } while (0)
+    AC_ADJ_Q(0);
+    AC_ADJ_Q(1);
+    AC_ADJ_Q(2);
+    AC_ADJ_Q(3);
+    AC_ADJ_Q(4);
+    AC_ADJ_Q(5);
+    AC_ADJ_Q(6);
+    if (entropy)
+        AC_ADJ_Q(7);
+    else
+        hipLaunchKernelGGL((adjoint_ss_passive_kernel<T>), bpg, tpb, 0, s, A);
+#undef AC_ADJ_Q
 #undef AC_ADJ
 }
 

@@ -233,7 +233,6 @@ void launch_reduce(const T* const* bufs, int map, int op, int nx, int ny, int nz
 }
 
 // ---------------------------------------------------------------- diagnostics
-constexpr int DIAG_SLOTS = 4 * NF + 4;
 
 template <typename T>
 struct DiagArgs {
@@ -245,15 +244,15 @@ struct DiagArgs {
 
 __device__ __forceinline__ int diag_op(int slot)
 {
-    // [min x8][max x8][sum sq x8][sum u2][max u2][sum b2][max b2]
+    // [min x8][max x8][sum sq x8] then, for |u|^2, |B|^2 and vA^2 = |B|^2 / rho: [sum][max][min]
     if (slot < NF)
         return 1;
     if (slot < 2 * NF)
         return 2;
     if (slot < 3 * NF)
         return 3;
-    const int r = slot - 3 * NF;
-    return (r & 1) ? 2 : 3;
+    const int r = (slot - 3 * NF) % 3;
+    return r == 0 ? 3 : (r == 1 ? 2 : 1);
 }
 
 template <typename T>
@@ -284,6 +283,7 @@ __global__ void __launch_bounds__(RED_TPB) diagnostics_stage1(DiagArgs<T> a)
         const double u2 = ux * ux + uy * uy + uz * uz;
         acc[3 * NF + 0] += u2;
         acc[3 * NF + 1] = fmax(acc[3 * NF + 1], u2);
+        acc[3 * NF + 2] = fmin(acc[3 * NF + 2], u2);
         const long long st[3] = {1, sy, sz};
         double g[3][3];
 #pragma unroll
@@ -299,8 +299,14 @@ __global__ void __launch_bounds__(RED_TPB) diagnostics_stage1(DiagArgs<T> a)
         }
         const double bx = g[2][1] - g[1][2], by = g[0][2] - g[2][0], bz = g[1][0] - g[0][1];
         const double b2 = bx * bx + by * by + bz * bz;
-        acc[3 * NF + 2] += b2;
-        acc[3 * NF + 3] = fmax(acc[3 * NF + 3], b2);
+        acc[3 * NF + 3] += b2;
+        acc[3 * NF + 4] = fmax(acc[3 * NF + 4], b2);
+        acc[3 * NF + 5] = fmin(acc[3 * NF + 5], b2);
+        // Alfven speed |B| / sqrt(rho), rho = exp(lnrho) (ref acc-runtime/stdlib/map.h:121-134)
+        const double va2 = b2 / exp((double)a.f[0][c]);
+        acc[3 * NF + 6] += va2;
+        acc[3 * NF + 7] = fmax(acc[3 * NF + 7], va2);
+        acc[3 * NF + 8] = fmin(acc[3 * NF + 8], va2);
     }
     for (int q = 0; q < DIAG_SLOTS; ++q) {
         const double r = block_reduce(diag_op(q), acc[q], sh);

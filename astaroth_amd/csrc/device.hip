@@ -44,7 +44,7 @@ static AcResult fail(const char* msg)
 
 namespace {
 constexpr int kTimingSlots = 4096;
-constexpr int kDiagSlots = 4 * NF + 4;
+constexpr int kDiagSlots = DIAG_SLOTS;
 }  // namespace
 
 struct device_s {
@@ -543,6 +543,73 @@ AcResult acDeviceLoadMesh(const Device d, const Stream stream, const AcMesh host
     return (AcResult)r;
 }
 
+extern "C++" {
+namespace acamd {
+void fourier_tables(const AcMeshInfo& info, uint64_t seed, int nmodes, int kmax, double ampl,
+                    std::vector<double>& head, std::vector<double>& tab);
+uint64_t lnrho_seed(uint64_t seed);
+}  // namespace acamd
+}
+
+AcResult acDeviceInitFourier(const Device d, const Stream stream, const unsigned long long seed,
+                             const int nmodes, const int kmax, const double ampl,
+                             const double lnrho_perturb)
+{
+    if (nmodes < 0 || kmax < 1)
+        return fail("acDeviceInitFourier: nmodes >= 0 and kmax >= 1 required");
+    AC_HIP_CHECK(hipSetDevice(d->id));
+    hipStream_t s = get_stream(d, stream);
+    const AcMeshInfo& info = d->local_config;
+    std::vector<double> head, tab;
+    fourier_tables(info, seed, nmodes, kmax, ampl, head, tab);
+    double *dhead = nullptr, *dtab = nullptr;
+    const size_t hb = std::max<size_t>(head.size(), 1) * sizeof(double);
+    const size_t tb = std::max<size_t>(tab.size(), 1) * sizeof(double);
+    if (hipMalloc(&dhead, hb) != hipSuccess || hipMalloc(&dtab, tb) != hipSuccess) {
+        hipFree(dhead);
+        return fail("acDeviceInitFourier: device allocation of the mode tables failed");
+    }
+    AC_HIP_CHECK(hipMemcpyAsync(dhead, head.data(), head.size() * sizeof(double),
+                                hipMemcpyHostToDevice, s));
+    AC_HIP_CHECK(hipMemcpyAsync(dtab, tab.data(), tab.size() * sizeof(double),
+                                hipMemcpyHostToDevice, s));
+    const int3 n = info.int3_params[AC_nlocal];
+    FourierArgs<AcReal> A;
+    for (int q = 0; q < 3; ++q) {
+        A.uu[q] = d->in[VTXBUF_UUX + q];
+        A.aa[q] = d->in[VTXBUF_AX + q];
+    }
+    A.lnrho = d->in[VTXBUF_LNRHO];
+    A.ss = d->in[VTXBUF_ENTROPY];
+    A.head = dhead;
+    A.tab = dtab;
+    A.nmodes = nmodes;
+    A.nx = n.x;
+    A.ny = n.y;
+    A.nz = n.z;
+    A.mx = n.x + 2 * R;
+    A.my = n.y + 2 * R;
+    A.mz = n.z + 2 * R;
+    A.lnrho_perturb = lnrho_perturb;
+    A.lseed = lnrho_seed(seed);
+    launch_fourier_ic<AcReal>(A, s);
+    AcReal* f[NF];
+    for (int q = 0; q < NF; ++q)
+        f[q] = d->in[q];
+    launch_periodic_fill<AcReal>(f, NF, n.x, n.y, n.z, s);
+    for (int q = 0; q < NF; ++q) {
+        launch_fill<AcReal>(d->out[q], d->count, (AcReal)0, s);
+        if (d->kahan_bits > 0)
+            launch_zero_bytes(d->comp[q], d->count * (size_t)(d->kahan_bits / 8), s);
+    }
+    AC_HIP_CHECK(hipGetLastError());
+    // the tables are consumed by the kernel above; free them once it has run
+    AC_HIP_CHECK(hipStreamSynchronize(s));
+    hipFree(dhead);
+    hipFree(dtab);
+    return AC_SUCCESS;
+}
+
 AcResult acDeviceSetVertexBuffer(const Device d, const Stream stream, const VertexBufferHandle h,
                                  const AcReal value)
 {
@@ -854,8 +921,13 @@ AcResult acDeviceReduceDiagnostics(const Device d, const Stream stream, AcDiagno
     }
     out->uu_rms = sqrt(v[3 * NF + 0] * inv_n);
     out->uu_max = sqrt(v[3 * NF + 1]);
-    out->bb_rms = sqrt(v[3 * NF + 2] * inv_n);
-    out->bb_max = sqrt(v[3 * NF + 3]);
+    out->uu_min = sqrt(v[3 * NF + 2]);
+    out->bb_rms = sqrt(v[3 * NF + 3] * inv_n);
+    out->bb_max = sqrt(v[3 * NF + 4]);
+    out->bb_min = sqrt(v[3 * NF + 5]);
+    out->va_rms = sqrt(v[3 * NF + 6] * inv_n);
+    out->va_max = sqrt(v[3 * NF + 7]);
+    out->va_min = sqrt(v[3 * NF + 8]);
     return AC_SUCCESS;
 }
 

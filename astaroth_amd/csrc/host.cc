@@ -214,6 +214,48 @@ void add_modes(const AcMeshInfo& info, const std::vector<Mode>& modes, double am
 
 }  // namespace
 
+namespace acamd {
+// Tables for evaluating the Fourier initial condition on a device exactly as add_modes does on
+// the host. Layout (doubles), for group g (0: uu, 1: aa) and mode j of that group:
+//   head[(g * nmodes + j) * 5 + {0..4}] = cos(phase), sin(phase), ampl e_x, ampl e_y, ampl e_z
+//   axis a in {x, y, z} with extent m_a: cos(k_a x_i), sin(k_a x_i) for i < m_a, stored at
+//   tab + ((g * nmodes + j) * (mx + my + mz) + off_a + i) * 2 + {0, 1}, off = (0, mx, mx + my).
+void fourier_tables(const AcMeshInfo& info, uint64_t seed, int nmodes, int kmax, double ampl,
+                    std::vector<double>& head, std::vector<double>& tab)
+{
+    const int3 m = info.int3_params[AC_mlocal];
+    const int3 nmin = info.int3_params[AC_nmin];
+    const AcReal3 ds = info.real3_params[AC_ds];
+    const double dsv[3] = {(double)ds.x, (double)ds.y, (double)ds.z};
+    const int mm[3] = {m.x, m.y, m.z};
+    const int n0[3] = {nmin.x, nmin.y, nmin.z};
+    const int off[3] = {0, m.x, m.x + m.y};
+    const size_t per = (size_t)(m.x + m.y + m.z);
+    head.assign((size_t)2 * nmodes * 5, 0.0);
+    tab.assign((size_t)2 * nmodes * per * 2, 0.0);
+    for (int g = 0; g < 2; ++g) {
+        const std::vector<Mode> modes = draw_modes(seed + (uint64_t)g, nmodes, kmax);
+        for (int j = 0; j < nmodes; ++j) {
+            const Mode& md = modes[j];
+            double* h = &head[((size_t)g * nmodes + j) * 5];
+            h[0] = std::cos(md.phase);
+            h[1] = std::sin(md.phase);
+            for (int q = 0; q < 3; ++q)
+                h[2 + q] = ampl * md.e[q];
+            double* t = &tab[((size_t)g * nmodes + j) * per * 2];
+            for (int d = 0; d < 3; ++d)
+                for (int i = 0; i < mm[d]; ++i) {
+                    const double x = (i - n0[d]) * dsv[d];
+                    t[(off[d] + i) * 2 + 0] = std::cos(md.k[d] * x);
+                    t[(off[d] + i) * 2 + 1] = std::sin(md.k[d] * x);
+                }
+        }
+    }
+}
+
+uint64_t lnrho_seed(uint64_t seed) { return (seed + 2ULL) * 0xD1B54A32D192ED03ULL; }
+}  // namespace acamd
+
 extern "C" {
 
 int acGetRealSize(void) { return (int)sizeof(AcReal); }
@@ -507,6 +549,58 @@ AcResult acHostMeshApplyPeriodicBounds(AcMesh* mesh)
     return AC_SUCCESS;
 }
 
+static const char* const kFieldNames[NUM_VTXBUF_HANDLES] = {
+    "VTXBUF_LNRHO", "VTXBUF_UUX", "VTXBUF_UUY", "VTXBUF_UUZ",
+    "VTXBUF_AX",    "VTXBUF_AY",  "VTXBUF_AZ",  "VTXBUF_ENTROPY"};
+
+const char* acGetVertexBufferName(const VertexBufferHandle handle)
+{
+    if ((int)handle < 0 || (int)handle >= NUM_VTXBUF_HANDLES)
+        return "";
+    return kFieldNames[handle];
+}
+
+// Same columns, order and number formatting as the diagnostics file the reference's standalone
+// driver appends to (ref samples/standalone_mpi/main.cc:218-391, "timeseries.ts"): header
+// "step  t_step  dt  uu_total_*  bb_total_*  vA_total_*  <field>_min  <field>_rms  <field>_max",
+// rows "%d %e %e " then "%e %e %e " per group.
+AcResult acHostAppendTimeseries(const char* path, const int step, const double t_step,
+                                const double dt, const AcDiagnostics* diag, int* found_nan)
+{
+    if (!path || !diag)
+        return AC_FAILURE;
+    FILE* fp = std::fopen(path, "a+");
+    if (!fp)
+        return AC_FAILURE;
+    std::fseek(fp, 0, SEEK_END);
+    if (std::ftell(fp) == 0) {
+        std::fprintf(fp, "step  t_step  dt  ");
+        std::fprintf(fp, "uu_total_min  uu_total_rms  uu_total_max  ");
+        std::fprintf(fp, "bb_total_min  bb_total_rms  bb_total_max  ");
+        std::fprintf(fp, "vA_total_min  vA_total_rms  vA_total_max  ");
+        for (int i = 0; i < NUM_VTXBUF_HANDLES; ++i)
+            std::fprintf(fp, "%s_min  %s_rms  %s_max  ", kFieldNames[i], kFieldNames[i],
+                         kFieldNames[i]);
+        std::fprintf(fp, "\n");
+    }
+    std::fprintf(fp, "%d %e %e ", step, t_step, dt);
+    std::fprintf(fp, "%e %e %e ", diag->uu_min, diag->uu_rms, diag->uu_max);
+    std::fprintf(fp, "%e %e %e ", diag->bb_min, diag->bb_rms, diag->bb_max);
+    std::fprintf(fp, "%e %e %e ", diag->va_min, diag->va_rms, diag->va_max);
+    int nan = 0;
+    for (int i = 0; i < NUM_VTXBUF_HANDLES; ++i) {
+        std::fprintf(fp, "%e %e %e ", diag->min[i], diag->rms[i], diag->max[i]);
+        if (std::isnan(diag->min[i]) || std::isnan(diag->rms[i]) || std::isnan(diag->max[i]))
+            nan = 1;
+    }
+    std::fprintf(fp, "\n");
+    const bool ok = std::fflush(fp) == 0;
+    std::fclose(fp);
+    if (found_nan)
+        *found_nan = nan;
+    return ok ? AC_SUCCESS : AC_FAILURE;
+}
+
 AcResult acHostMeshInitFourier(AcMesh* mesh, const unsigned long long seed, const int nmodes,
                                const int kmax, const double ampl, const double lnrho_perturb)
 {
@@ -527,7 +621,7 @@ AcResult acHostMeshInitFourier(AcMesh* mesh, const unsigned long long seed, cons
     }
     for (size_t i = 0; i < count; ++i)
         mesh->vertex_buffer[VTXBUF_ENTROPY][i] = (AcReal)0;
-    const uint64_t lseed = (seed + 2ULL) * 0xD1B54A32D192ED03ULL;
+    const uint64_t lseed = acamd::lnrho_seed(seed);
     for (size_t i = 0; i < count; ++i) {
         double v = 0.0;
         if (lnrho_perturb != 0.0) {

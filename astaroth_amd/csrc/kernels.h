@@ -70,6 +70,27 @@ int tiled_num_shapes();
 TiledShape tiled_shape(int i);
 bool tiled_shape_fits(TiledShape s, int nx, int ny, int nz);
 
+// ---- seeded Fourier-mode initial condition (ic.hip) ----
+template <typename T>
+struct FourierArgs {
+    T* uu[3];
+    T* aa[3];
+    T* lnrho;
+    T* ss;
+    const double* head;  // per (group, mode): cos phase, sin phase, ampl * e (3)
+    const double* tab;   // per (group, mode): cos / sin of k_a x_a along x, y, z
+    int nmodes;
+    int nx, ny, nz, mx, my, mz;
+    double lnrho_perturb;
+    unsigned long long lseed;
+};
+template <typename T>
+void launch_fourier_ic(const FourierArgs<T>& A, hipStream_t s);
+
+// ---- one-pass diagnostics: per field min, max, sum of squares; then sum, max and min of
+// |u|^2, |curl A|^2 and |curl A|^2 / exp(lnrho)
+constexpr int DIAG_SLOTS = 3 * NF + 9;
+
 // ---- periodic fill: every halo cell of every listed field, one launch ----
 template <typename T>
 void launch_periodic_fill(T* const* fields, int nfields, int nx, int ny, int nz, hipStream_t s);

@@ -200,6 +200,262 @@ def adjoint_cost(ac, n, steps=5):
         runner.close()
 
 
+# ---------------------------------------------------------------------------------------------
+# Measurement forms beyond the headline line (python bench.py --form NAME); each prints one JSON
+# line. The GPU's own FP64 mode is the reference for the precision comparisons.
+
+PHYS = dict(AC_len=(2 * np.pi,) * 3, AC_cs_sound=1.0, AC_gamma=5.0 / 3.0, AC_nu_visc=5e-3,
+            AC_eta=5e-3, AC_chi=5e-3, AC_K_heatcond=5e-3, AC_lupwd=True, AC_lentropy=True)
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def state_tensor(lib, dev):
+    """The device state (halos filled) copied into a torch tensor on the same GPU."""
+    torch = _torch()
+    mx, my, mz = dev.info["AC_mlocal"]
+    t = torch.empty((NF, mz, my, mx), dtype=torch.float32 if lib.precision == "fp32" else
+                    torch.float64, device="cuda")
+    dev.periodic()
+    dev.sync()
+    torch.cuda.synchronize()
+    if lib.acDeviceStoreMeshToDevicePtr(dev.handle, 0, ctypes.c_void_p(t.data_ptr())):
+        raise RuntimeError("acDeviceStoreMeshToDevicePtr failed")
+    dev.sync()
+    return t
+
+
+def field_errors_gpu(ref, cand):
+    """Per field: max |cand - ref| / max |ref| and the relative L2 error, over interior cells."""
+    out = []
+    for q in range(NF):
+        m = ref[q, 3:-3, 3:-3, 3:-3].double()
+        d = cand[q, 3:-3, 3:-3, 3:-3].double() - m
+        scale = max(float(m.abs().max()), 1e-300)
+        out.append({"maxrel": float(d.abs().max()) / scale,
+                    "l2": float(d.pow(2).sum().sqrt()) / max(float(m.pow(2).sum().sqrt()), 1e-300)})
+        del m, d
+    return out
+
+
+def fixed_steps(dev, dt, steps):
+    dev.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dev.step(dt)
+    dev.sync()
+    return time.perf_counter() - t0
+
+
+def form_precision(ac, n=512, steps=200, mid=100, seed=512, kbs=(8, 16, 32)):
+    """512^3, 64 modes |k| <= 8, amplitude 0.1, lnrho + U(-1e-4, 1e-4); FP64 reference, plain FP32
+    and Kahan-compensated FP32 (compensation stored in 8, 16 or 32 bits). This initial condition
+    is magnetically dominated (Alfven speed well above cs + |u|), so dt is recomputed every step
+    from the FP64 state as 0.2 dx / (cs + max|u| + max v_A) and the same dt sequence is replayed
+    in every FP32 variant. Errors at `mid` and `steps` steps."""
+    torch = _torch()
+    n3 = n ** 3
+    dx = 2 * np.pi / n
+    l64 = ac.load("fp64")
+    info64 = l64.config((n, n, n), **PHYS)
+    dev = ac.Device(l64, info64)
+    dev.init_fourier(seed, 64, 8, 0.1, 1e-4)
+    dts, vmax = [], []
+    t_ref = 0.0
+    refs = {}
+    for k in range(steps):
+        dev.periodic()
+        d = dev.diagnostics()
+        dt = 0.2 * dx / (1.0 + d.uu_max + d.va_max)
+        dts.append(dt)
+        vmax.append((d.uu_max, d.va_max))
+        dev.sync()
+        t0 = time.perf_counter()
+        dev.step(l64.np_real(dt))
+        dev.sync()
+        t_ref += time.perf_counter() - t0
+        if k + 1 in (mid, steps):
+            refs[k + 1] = state_tensor(l64, dev)
+    dev.close()
+    runs = {"fp64": {"cell_updates_per_s": n3 * steps / t_ref}}
+    for kb in (0,) + tuple(kbs):
+        l32 = ac.load("fp32")
+        dv = ac.Device(l32, l32.config((n, n, n), AC_kahan_bits=kb, **PHYS))
+        dv.init_fourier(seed, 64, 8, 0.1, 1e-4)
+        t_run = 0.0
+        errs = {}
+        for k in range(steps):
+            dv.sync()
+            t0 = time.perf_counter()
+            dv.step(l32.np_real(dts[k]))
+            dv.sync()
+            t_run += time.perf_counter() - t0
+            if k + 1 in refs:
+                errs[k + 1] = field_errors_gpu(refs[k + 1], state_tensor(l32, dv))
+        dv.close()
+        torch.cuda.empty_cache()
+        rec = {"cell_updates_per_s": n3 * steps / t_run}
+        for k, e in errs.items():
+            rec[f"errors_after_{k}"] = {ac.FIELD_NAMES[q]: e[q] for q in range(NF)}
+            rec[f"max_maxrel_after_{k}"] = max(x["maxrel"] for x in e)
+        runs["fp32" if kb == 0 else f"fp32_kahan{kb}"] = rec
+    return {"form": "precision", "grid": [n, n, n], "steps": steps,
+            "dt_first_last": [dts[0], dts[-1]], "max_u_first_last": [vmax[0][0], vmax[-1][0]],
+            "max_vA_first_last": [vmax[0][1], vmax[-1][1]],
+            "ic": "64 modes |k|<=8 ampl 0.1, lnrho U(-1e-4,1e-4), seed %d" % seed,
+            "timing": "wall time of the steps only, device-synchronised per step",
+            "reference": "FP64 GPU mode", "runs": runs}
+
+
+def form_scale(ac, n=1024, steps=50, kb=16, seed=1024):
+    """1024^3 FP32 with Kahan compensation (kb-bit), 256 modes |k| <= 16, amplitude 0.1, device
+    CFL dt, autotuned launch shape, initial condition generated on the device."""
+    lib = ac.load("fp32")
+    info = lib.config((n, n, n), AC_kahan_bits=kb, **PHYS)
+    t0 = time.perf_counter()
+    dev = ac.Device(lib, info)
+    dev.init_fourier(seed, 256, 16, 0.1, 0.0)
+    t_ic = time.perf_counter() - t0
+    wall, solve_ms, launches = timed(lib, dev, steps, 3)
+    tx, ty, tz = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    lib.acDeviceGetSolveLaunchShape(dev.handle, ctypes.byref(tx), ctypes.byref(ty), ctypes.byref(tz))
+    dev.close()
+    m3 = (n + 6) ** 3
+    resident = m3 * NF * (3 * 4 + kb // 8)
+    bl = per_launch_bytes(n ** 3, 4, kb)
+    ach = bl / (solve_ms / 1e3 / max(launches, 1)) / 1e9
+    return {"form": "scale", "grid": [n, n, n], "dtype": "f32", "kahan_bits": kb, "steps": steps,
+            "cell_updates_per_s": n ** 3 * steps / wall, "ms_per_step": wall / steps * 1e3,
+            "resident_bytes": resident, "ic_seconds": t_ic,
+            "launch_shape": [tx.value, ty.value, tz.value],
+            "solve_achieved_GBs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4)}
+
+
+def form_scale_adjoint(ac, n=384, steps=500, snaps=24, seed=384):
+    """Adjoint of a 384^3 FP32 run of 500 fixed-dt steps with revolve checkpointing (snapshot
+    slots sized to the HBM budget), loss 0.5 |u(T) - u_target|^2 with u_target the initial u of a
+    second seed; reports forward, reverse and recomputation cost."""
+    torch = _torch()
+    from astaroth_amd import adjoint as AD
+    lib = ac.load("fp32")
+    info = lib.config((n, n, n), **PHYS)
+    runner = AD.MHDAdjoint(lib, info, steps, 1.0, schedule="revolve", snaps=snaps)
+    try:
+        runner.dev.init_fourier(seed, 256, 16, 0.1, 0.0)
+        umax = runner.dev.reduce_vec("RTYPE_MAX", 1, 2, 3)
+        runner.dt = lib.np_real(0.2 * (2 * np.pi / n) / (1.0 + umax))
+        f0 = state_tensor(lib, runner.dev)
+        runner.dev.init_fourier(seed + 7, 256, 16, 0.1, 0.0)
+        tgt = state_tensor(lib, runner.dev)[1:4].clone()
+        f0.requires_grad_(True)
+        nu = torch.tensor(5e-3, dtype=torch.float32, requires_grad=True)
+        eta = torch.tensor(5e-3, dtype=torch.float32, requires_grad=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = AD.MHDSolve.apply(f0, nu, eta, runner)
+        torch.cuda.synchronize()
+        t_fwd = time.perf_counter() - t0
+        fwd_steps = runner.forward_steps
+        loss = 0.5 * ((out[1:4, 3:-3, 3:-3, 3:-3] - tgt[:, 3:-3, 3:-3, 3:-3]).double() ** 2).sum()
+        t0 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t_bwd = time.perf_counter() - t0
+        rec = runner.forward_steps - fwd_steps
+        return {"form": "scale_adjoint", "grid": [n, n, n], "dtype": "f32", "steps": steps,
+                "schedule": f"revolve, {snaps} snapshot slots of {NF * (n + 6) ** 3 * 4} bytes",
+                "forward_s": t_fwd, "reverse_s": t_bwd,
+                "reverse_over_forward": t_bwd / t_fwd,
+                "recomputed_steps": rec, "recomputed_per_step": rec / steps,
+                "loss": float(loss), "grad_nu": float(nu.grad), "grad_eta": float(eta.grad)}
+    finally:
+        runner.close()
+
+
+def form_differentiation(ac, n=128, steps=50, every=10, seed=128, ndir=8):
+    """128^3, 50 fixed-dt steps, checkpoints every 10 steps; loss 0.5 |u(T) - u_target|^2 with
+    u_target the state of a second seeded run. FP32 adjoint gradient, FP64 adjoint gradient, and
+    centred differences of the FP64 loss (h = 1e-3 without upwinding, where the loss is smooth;
+    h = 1e-5 with it) on ndir random directions plus nu and eta."""
+    torch = _torch()
+    from astaroth_amd import adjoint as AD
+    res = {"form": "differentiation", "grid": [n, n, n], "steps": steps,
+           "checkpoint_every": every}
+    dx = 2 * np.pi / n
+    dt = 0.2 * dx / 1.2
+    res["dt"] = dt
+    gen = torch.Generator(device="cpu").manual_seed(seed)
+    for upwd in (False, True):
+        out = {}
+        grads = {}
+        for prec in ("fp64", "fp32"):
+            lib = ac.load(prec)
+            info = lib.config((n, n, n), **dict(PHYS, AC_lupwd=upwd))
+            runner = AD.MHDAdjoint(lib, info, steps, dt, schedule="uniform", every=every)
+            try:
+                runner.dev.init_fourier(seed + 1, 16, 4, 0.05, 1e-4)
+                runner.advance(steps)
+                tgt = state_tensor(lib, runner.dev)[1:4].clone()
+                runner.dev.init_fourier(seed, 16, 4, 0.05, 1e-4)
+                f0 = state_tensor(lib, runner.dev)
+                ft = f0.clone().requires_grad_(True)
+                nu = torch.tensor(5e-3, dtype=runner.tdtype, requires_grad=True)
+                eta = torch.tensor(5e-3, dtype=runner.tdtype, requires_grad=True)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                y = AD.MHDSolve.apply(ft, nu, eta, runner)
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+                L = 0.5 * ((y[1:4, 3:-3, 3:-3, 3:-3] - tgt[:, 3:-3, 3:-3, 3:-3]).double() ** 2).sum()
+                L.backward()
+                torch.cuda.synchronize()
+                t2 = time.perf_counter()
+                grads[prec] = (ft.grad.double(), float(nu.grad), float(eta.grad))
+                out[prec] = {"forward_s": t1 - t0, "backward_s": t2 - t1,
+                             "backward_over_forward": (t2 - t1) / (t1 - t0)}
+                if prec == "fp64":
+                    f0_64, tgt_64, runner64 = f0, tgt, runner
+                    continue
+            finally:
+                if prec != "fp64":
+                    runner.close()
+
+        def loss64(f, nu_v, eta_v):
+            y, _ = runner64.run_forward(f, nu_v, eta_v)
+            return 0.5 * float(((y[1:4, 3:-3, 3:-3, 3:-3] - tgt_64[:, 3:-3, 3:-3, 3:-3]) ** 2).sum())
+
+        h = 1e-5 if upwd else 1e-3
+        rows = []
+        for k in range(ndir):
+            v = torch.zeros_like(f0_64)
+            v[:, 3:-3, 3:-3, 3:-3] = 0.01 * torch.randn(v[:, 3:-3, 3:-3, 3:-3].shape,
+                                                        generator=gen, dtype=torch.float64).cuda()
+            fd = (loss64(f0_64 + h * v, 5e-3, 5e-3) - loss64(f0_64 - h * v, 5e-3, 5e-3)) / (2 * h)
+            a64 = float((grads["fp64"][0] * v).sum())
+            a32 = float((grads["fp32"][0] * v).sum())
+            rows.append({"fd": fd, "rel_fp64": abs(a64 - fd) / abs(fd),
+                         "rel_fp32": abs(a32 - fd) / abs(fd)})
+        for name, idx in (("nu", 1), ("eta", 2)):
+            hp = 1e-3 * 5e-3
+            a = (5e-3 + hp, 5e-3) if name == "nu" else (5e-3, 5e-3 + hp)
+            b = (5e-3 - hp, 5e-3) if name == "nu" else (5e-3, 5e-3 - hp)
+            fd = (loss64(f0_64, *a) - loss64(f0_64, *b)) / (2 * hp)
+            rows.append({"param": name, "fd": fd,
+                         "rel_fp64": abs(grads["fp64"][idx] - fd) / abs(fd),
+                         "rel_fp32": abs(grads["fp32"][idx] - fd) / abs(fd)})
+        runner64.close()
+        out["fd_step"] = h
+        out["checks"] = rows
+        out["max_rel_fp64"] = max(r["rel_fp64"] for r in rows)
+        out["max_rel_fp32"] = max(r["rel_fp32"] for r in rows)
+        res["upwind" if upwd else "no_upwind"] = out
+        torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -210,6 +466,8 @@ def main():
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--variant-steps", type=int, default=30)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--form", choices=["precision", "scale", "scale_adjoint", "differentiation"],
+                    help="run one measurement form instead of the headline benchmark")
     ap.add_argument("--shape", type=int, nargs=3, default=(0, 0, 0),
                     help="pin the solve launch shape TX TY ZC (0 0 0 = autotune)")
     ap.add_argument("--copy-calibration", type=int, default=0,
@@ -222,6 +480,11 @@ def main():
         sys.exit(2)
 
     from astaroth_amd import astaroth as ac
+    if args.form:
+        fn = {"precision": form_precision, "scale": form_scale,
+              "scale_adjoint": form_scale_adjoint, "differentiation": form_differentiation}
+        print(json.dumps(fn[args.form](ac)), flush=True)
+        return
     n = args.n
     n3 = n ** 3
     lib, dev = make_device(ac, "fp32", n, 0, tuple(args.shape))

@@ -247,6 +247,11 @@ typedef struct {
     double uu_max;   /* max |u| */
     double bb_rms;   /* sqrt(<|curl A|^2>) */
     double bb_max;   /* max |curl A| */
+    double uu_min;   /* min |u| */
+    double bb_min;   /* min |curl A| */
+    double va_min;   /* Alfven speed |curl A| / sqrt(exp(lnrho)): min, rms, max */
+    double va_rms;
+    double va_max;
 } AcDiagnostics;
 
 AC_BEGIN_C_DECLARATIONS
@@ -412,6 +417,14 @@ AcResult acDeviceGetStreamHandle(const Device device, const Stream stream, void*
 AcResult acDeviceLoadMeshFromDevicePtr(const Device device, const Stream stream, const AcReal* src);
 AcResult acDeviceStoreMeshToDevicePtr(const Device device, const Stream stream, AcReal* dst);
 
+/* The seeded Fourier-mode initial condition of acHostMeshInitFourier, generated directly in the
+ * device's input buffers (bit-identical to the host version followed by acDeviceLoadMesh), with
+ * halos filled and the output and compensation buffers cleared. For grids whose host mesh is
+ * impractical to build (1024^3 and up). No reference counterpart. Synchronises the stream. */
+AcResult acDeviceInitFourier(const Device device, const Stream stream, const unsigned long long seed,
+                             const int nmodes, const int kmax, const double ampl,
+                             const double lnrho_perturb);
+
 /* ---- Differentiation of one RK3 substep (no reference counterpart) ----
  * Linearisation point: the device's current state `in` (halos filled by the caller, as before
  * acDeviceIntegrateSubstep). Slabs hold NUM_VTXBUF_HANDLES fields of m^3 values each.
@@ -461,6 +474,16 @@ AcResult acHostMeshApplyPeriodicBounds(AcMesh* mesh);
  * Uses SplitMix64 (seed, seed+1, seed+2 for uu, aa, lnrho). FP64 arithmetic, cast to AcReal. */
 AcResult acHostMeshInitFourier(AcMesh* mesh, const unsigned long long seed, const int nmodes,
                                const int kmax, const double ampl, const double lnrho_perturb);
+
+/* Name of a field as the reference prints it ("VTXBUF_LNRHO", ...; ref vtxbuf_names). */
+const char* acGetVertexBufferName(const VertexBufferHandle handle);
+
+/* Appends one row of diagnostics to a timeseries file in the format of the reference's
+ * standalone driver (ref samples/standalone_mpi/main.cc:218-391, timeseries.ts), writing the
+ * header first when the file is new or empty. *found_nan (if given) is set when a per-field
+ * reduction is NaN, the condition on which that driver stops. */
+AcResult acHostAppendTimeseries(const char* path, const int step, const double t_step,
+                                const double dt, const AcDiagnostics* diag, int* found_nan);
 
 /* Library identification: sizeof(AcReal), sizeof(AcMeshInfo), sizeof(AcMesh) of this build and
  * the offload target it was built for (a caller checks its own struct sizes against these, as

@@ -303,3 +303,57 @@ def test_beltrami_closed_form_large(lib):
     np.testing.assert_allclose(cand[6][sl], amp * data[6][sl], atol=tol * 0.1)
     for q in (1, 2, 3, 4):
         assert np.max(np.abs(cand[q][sl])) < tol * 0.1
+
+
+def curl_6th(a, ds):
+    """curl of (a[0], a[1], a[2]) on interior cells with the 6th-order central first derivative
+    (3/4, -3/20, 1/60) / ds, halos periodic-filled."""
+    c = (0.75, -0.15, 1.0 / 60.0)
+
+    def d(f, axis, h):
+        n = f.shape[axis] - 6
+        out = 0.0
+        for k in (1, 2, 3):
+            hi = np.take(f, range(3 + k, 3 + k + n), axis=axis)
+            lo = np.take(f, range(3 - k, 3 - k + n), axis=axis)
+            out = out + c[k - 1] * (hi - lo)
+        sl = [slice(3, -3)] * 3
+        sl[axis] = slice(None)
+        return out[tuple(sl)] / h
+
+    ax, ay, az = (x.astype(np.float64) for x in a)
+    # arrays are (z, y, x): axis 2 is x, 1 is y, 0 is z
+    bx = d(az, 1, ds[1]) - d(ay, 0, ds[2])
+    by = d(ax, 0, ds[2]) - d(az, 2, ds[0])
+    bz = d(ay, 2, ds[0]) - d(ax, 1, ds[1])
+    return bx, by, bz
+
+
+def test_diagnostics_magnetic_and_alfven(lib):
+    """|curl A| and the Alfven speed |B| / sqrt(exp(lnrho)) of the one-pass diagnostics against
+    a numpy 6th-order curl of the same data; min |u| as well."""
+    say(f"case diagnostics B / vA {lib.precision}")
+    info = lib.config((24, 20, 16), AC_len=(2 * np.pi,) * 3, AC_cs_sound=1.0,
+                      AC_gamma=5.0 / 3.0, AC_nu_visc=5e-3, AC_eta=5e-3, AC_chi=5e-3,
+                      AC_K_heatcond=5e-3)
+    data = c1_initial(lib, info, seed=13)
+    data[0] += 0.05 * np.cos(data[4] * 7.0).astype(data.dtype)
+    O.periodic_fill(data, O.config_from_info(info))
+    mesh = ac.HostMesh(lib, info, data)
+    with ac.Device(lib, info) as dev:
+        dev.load_mesh(mesh)
+        dev.periodic()
+        d = dev.diagnostics()
+    ds = [float(v) for v in info["AC_ds"]]
+    bx, by, bz = curl_6th(data[4:7], ds)
+    b2 = bx * bx + by * by + bz * bz
+    sl = O.interior_slices(data.shape[1:])
+    va2 = b2 / np.exp(data[0][sl].astype(np.float64))
+    u2 = sum(data[q][sl].astype(np.float64) ** 2 for q in (1, 2, 3))
+    tol = 2e-5 if lib.precision == "fp32" else 1e-11
+    for name, got, want in (("bb_max", d.bb_max, np.sqrt(b2.max())), ("bb_rms", d.bb_rms, np.sqrt(b2.mean())),
+                            ("bb_min", d.bb_min, np.sqrt(b2.min())), ("va_max", d.va_max, np.sqrt(va2.max())),
+                            ("va_rms", d.va_rms, np.sqrt(va2.mean())), ("va_min", d.va_min, np.sqrt(va2.min())),
+                            ("uu_min", d.uu_min, np.sqrt(u2.min()))):
+        say(f"  {name}: {got:.9g} vs {want:.9g}")
+        assert got == pytest.approx(want, rel=tol, abs=tol * np.sqrt(b2.max())), name
