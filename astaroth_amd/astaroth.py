@@ -19,6 +19,8 @@ LIBDIR = os.path.join(HERE, "lib")
 # ---- enums (same values as include/astaroth.h) ----
 (VTXBUF_LNRHO, VTXBUF_UUX, VTXBUF_UUY, VTXBUF_UUZ, VTXBUF_AX, VTXBUF_AY, VTXBUF_AZ,
  VTXBUF_ENTROPY, NUM_VTXBUF_HANDLES) = range(9)
+AC_TIMESTEP_COURANT, AC_TIMESTEP_REFERENCE = 0, 1
+
 FIELD_NAMES = ["VTXBUF_LNRHO", "VTXBUF_UUX", "VTXBUF_UUY", "VTXBUF_UUZ", "VTXBUF_AX",
                "VTXBUF_AY", "VTXBUF_AZ", "VTXBUF_ENTROPY"]
 
@@ -88,7 +90,7 @@ EXPORTED = [
     "acGetBuildInfo",
     "acAdjointWorkspaceBytes", "acDeviceIntegrateSubstepTangent", "acDeviceIntegrateSubstepAdjoint",
     "acDeviceLoadMeshFromDevicePtr", "acDeviceStoreMeshToDevicePtr", "acDeviceInitFourier",
-    "acHostAppendTimeseries", "acGetVertexBufferName",
+    "acHostAppendTimeseries", "acGetVertexBufferName", "acDeviceSetTimestepMode",
 ]
 
 
@@ -282,6 +284,7 @@ class AstarothLib:
             "acDeviceStoreMeshToDevicePtr": [dev, i, ctypes.c_void_p],
             "acDeviceInitFourier": [dev, i, ctypes.c_ulonglong, i, i, ctypes.c_double,
                                     ctypes.c_double],
+            "acDeviceSetTimestepMode": [dev, i],
         }
         sig["acHostAppendTimeseries"] = [ctypes.c_char_p, i, ctypes.c_double, ctypes.c_double,
                                          ctypes.POINTER(AcDiagnostics), ctypes.POINTER(ctypes.c_int)]

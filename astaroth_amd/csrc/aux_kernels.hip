@@ -397,6 +397,81 @@ void launch_umax2(const T* const* uu, int nx, int ny, int nz, unsigned long long
     hipLaunchKernelGGL(umax2_kernel<T>, dim3(nb), dim3(RED_TPB), 0, s, a);
 }
 
+// ---------------------------------------------------------------- reference timestep inputs
+// The (x, y) pair of cfl_cell for the current state (modes 1 and 2): |u|^2 and the Alfven speed
+// squared |curl A|^2 / exp(lnrho), or (|u| + sqrt(cs^2 + vA^2))^2; maxima over interior cells.
+template <typename T>
+struct CflArgs {
+    const T* f[NF];
+    Coeffs<T> C;
+    int nx, ny, nz, mx, my;
+    int mode;
+    double cs2;
+    unsigned long long* out;  // pair
+};
+
+template <typename T>
+__global__ void __launch_bounds__(RED_TPB) cfl_state_kernel(CflArgs<T> a)
+{
+    const long long n = (long long)a.nx * a.ny * a.nz;
+    const long long sy = a.mx, sz = (long long)a.mx * a.my;
+    double mx_ = 0.0, my_ = 0.0;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % a.nx);
+        const long long r = t / a.nx;
+        const int j = (int)(r % a.ny);
+        const int k = (int)(r / a.ny);
+        const long long c = (long long)(i + R) + sy * (j + R) + sz * (k + R);
+        const long long st[3] = {1, sy, sz};
+        CellDerivs<T> d;
+        d.lr = a.f[0][c];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            d.u[q] = a.f[1 + q][c];
+            const T* f = a.f[4 + q];
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) {
+                const long long s = st[ax];
+                const T* kk = a.C.d1[ax];
+                d.ga[q][ax] = kk[1] * (f[c + s] - f[c - s]) + kk[2] * (f[c + 2 * s] - f[c - 2 * s]) +
+                              kk[3] * (f[c + 3 * s] - f[c - 3 * s]);
+            }
+        }
+        double x, y;
+        cfl_cell<T>(a.mode, a.cs2, d, x, y);
+        mx_ = fmax(mx_, x);
+        my_ = fmax(my_, y);
+    }
+    wave_max_to_global(mx_, a.out);
+    if (a.mode == 1)
+        wave_max_to_global(my_, a.out + 1);
+}
+
+template <typename T>
+void launch_cfl_state(const T* const* fields, const Coeffs<T>& C, int nx, int ny, int nz, int mode,
+                      double cs2, unsigned long long* out, hipStream_t s)
+{
+    CflArgs<T> a;
+    for (int q = 0; q < NF; ++q)
+        a.f[q] = fields[q];
+    a.C = C;
+    a.nx = nx;
+    a.ny = ny;
+    a.nz = nz;
+    a.mx = nx + 2 * R;
+    a.my = ny + 2 * R;
+    a.mode = mode;
+    a.cs2 = cs2;
+    a.out = out;
+    const long long n = (long long)nx * ny * nz;
+    const int nb = (int)std::min<long long>((n + RED_TPB - 1) / RED_TPB, 4096);
+    hipLaunchKernelGGL(cfl_state_kernel<T>, dim3(nb), dim3(RED_TPB), 0, s, a);
+}
+
+template void launch_cfl_state<AcRealT>(const AcRealT* const*, const Coeffs<AcRealT>&, int, int, int,
+                                         int, double, unsigned long long*, hipStream_t);
+
 // ---------------------------------------------------------------- fills
 template <typename T>
 __global__ void fill_kernel(T* dst, size_t n, T v)

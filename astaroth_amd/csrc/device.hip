@@ -67,10 +67,11 @@ struct device_s {
     double* partials;
     int max_partials;
     double* dres;                 // [0] reduce result, [1..] diagnostics
-    unsigned long long* umax2;    // two slots
+    unsigned long long* umax2;    // two slot pairs (x, y) of the timestep reductions
     double* time_acc;
     int cfl_cur;
     double cfl_courant;
+    int cfl_mode;                 // 0 courant, 1 reference (2 when AC_additive_timestep)
     double* host_pinned;          // staging for small D2H reads
     // launch shape
     TiledShape shape;
@@ -144,6 +145,25 @@ static void make_phys(const AcMeshInfo& info, PhysParams<AcReal>& P)
     P.nu_third = (AcReal)((double)info.real_params[AC_nu_visc] / 3.0);
 }
 
+// Parameters of the device timestep formula (cfl_dt) for the device's timestep mode. In the
+// reference modes the Courant factors are AC_cdt and AC_cdtv of the configuration and the
+// diffusive limit uses max(nu, eta) (default CFL: its chi term is 0, ref
+// samples/standalone_mpi/main.cc:413) or max(nu, eta, gamma chi) (additive, ref mhdsolver.ac:560).
+static void set_cfl_args(Device d, SolveArgs<AcReal>& a)
+{
+    const AcMeshInfo& info = d->local_config;
+    const double nu = (double)info.real_params[AC_nu_visc], eta = (double)info.real_params[AC_eta];
+    a.cfl_mode = d->cfl_mode;
+    a.cfl_cs2 = (double)info.real_params[AC_cs_sound] * (double)info.real_params[AC_cs_sound];
+    a.cfl_cdtv = (double)info.real_params[AC_cdtv];
+    double visc = std::max(nu, eta);
+    if (d->cfl_mode == 2)
+        visc = std::max(visc, (double)info.real_params[AC_gamma] * (double)info.real_params[AC_chi]);
+    a.cfl_visc = visc;
+    if (d->cfl_mode != 0)
+        a.cfl_cdt = (double)info.real_params[AC_cdt];
+}
+
 static void fill_solve_args(Device d, SolveArgs<AcReal>& a, int step, AcReal dt)
 {
     const AcMeshInfo& info = d->local_config;
@@ -176,6 +196,7 @@ static void fill_solve_args(Device d, SolveArgs<AcReal>& a, int step, AcReal dt)
     a.cfl_cdt = 0.0;
     a.cfl_dsmin = (double)info.real_params[AC_dsmin];
     a.cfl_cs = (double)info.real_params[AC_cs_sound];
+    set_cfl_args(d, a);
 }
 
 static AcResult launch_solve(Device d, const SolveArgs<AcReal>& a, hipStream_t s)
@@ -277,9 +298,9 @@ AcResult acDeviceCreate(const int id, const AcMeshInfo device_config, Device* de
     d->max_partials = std::max(reduce_num_partials(n.x, n.y, n.z), 1) * kDiagSlots;
     AC_HIP_CHECK(hipMalloc(&d->partials, sizeof(double) * d->max_partials));
     AC_HIP_CHECK(hipMalloc(&d->dres, sizeof(double) * (1 + kDiagSlots)));
-    AC_HIP_CHECK(hipMalloc(&d->umax2, sizeof(unsigned long long) * 2));
+    AC_HIP_CHECK(hipMalloc(&d->umax2, sizeof(unsigned long long) * 4));
     AC_HIP_CHECK(hipMalloc(&d->time_acc, sizeof(double)));
-    launch_zero_bytes(d->umax2, sizeof(unsigned long long) * 2, s0);
+    launch_zero_bytes(d->umax2, sizeof(unsigned long long) * 4, s0);
     launch_zero_bytes(d->time_acc, sizeof(double), s0);
     AC_HIP_CHECK(hipHostMalloc((void**)&d->host_pinned, sizeof(double) * (2 + kDiagSlots)));
 
@@ -818,15 +839,36 @@ AcResult acDeviceReduceVecScal(const Device d, const Stream stream, const AcRedu
 
 // ------------------------------------------------------------------ extensions
 
+AcResult acDeviceSetTimestepMode(const Device d, const int mode)
+{
+    if (mode != AC_TIMESTEP_COURANT && mode != AC_TIMESTEP_REFERENCE)
+        return fail("acDeviceSetTimestepMode: unknown mode");
+    d->cfl_mode = mode == AC_TIMESTEP_COURANT
+                      ? 0
+                      : (d->local_config.bool_params[AC_additive_timestep] ? 2 : 1);
+    return AC_SUCCESS;
+}
+
 AcResult acDeviceUpdateTimestepCFL(const Device d, const Stream stream, const AcReal courant)
 {
     AC_HIP_CHECK(hipSetDevice(d->id));
     hipStream_t s = get_stream(d, stream);
     const int3 n = d->local_config.int3_params[AC_nlocal];
-    unsigned long long* slot = d->umax2 + d->cfl_cur;
-    launch_zero_bytes(slot, sizeof(unsigned long long), s);
-    const AcReal* uu[3] = {d->in[VTXBUF_UUX], d->in[VTXBUF_UUY], d->in[VTXBUF_UUZ]};
-    launch_umax2<AcReal>(uu, n.x, n.y, n.z, slot, s);
+    unsigned long long* slot = d->umax2 + 2 * d->cfl_cur;
+    launch_zero_bytes(slot, 2 * sizeof(unsigned long long), s);
+    if (d->cfl_mode == 0) {
+        const AcReal* uu[3] = {d->in[VTXBUF_UUX], d->in[VTXBUF_UUY], d->in[VTXBUF_UUZ]};
+        launch_umax2<AcReal>(uu, n.x, n.y, n.z, slot, s);
+    }
+    else {
+        Coeffs<AcReal> C;
+        make_coeffs(d->local_config, C);
+        const AcReal* f[NF];
+        for (int q = 0; q < NF; ++q)
+            f[q] = d->in[q];
+        const double cs = (double)d->local_config.real_params[AC_cs_sound];
+        launch_cfl_state<AcReal>(f, C, n.x, n.y, n.z, d->cfl_mode, cs * cs, slot, s);
+    }
     AC_HIP_CHECK(hipGetLastError());
     d->cfl_courant = (double)courant;
     return AC_SUCCESS;
@@ -836,13 +878,18 @@ AcResult acDeviceGetTimestep(const Device d, const Stream stream, double* dt)
 {
     AC_HIP_CHECK(hipSetDevice(d->id));
     hipStream_t s = get_stream(d, stream);
-    AC_HIP_CHECK(hipMemcpyAsync(d->host_pinned, d->umax2 + d->cfl_cur, sizeof(double),
+    AC_HIP_CHECK(hipMemcpyAsync(d->host_pinned, d->umax2 + 2 * d->cfl_cur, 2 * sizeof(double),
                                 hipMemcpyDeviceToHost, s));
     AC_HIP_CHECK(hipStreamSynchronize(s));
-    const double umax = sqrt(d->host_pinned[0]);
+    SolveArgs<AcReal> a;
+    memset(&a, 0, sizeof(a));
     const AcMeshInfo& info = d->local_config;
-    *dt = d->cfl_courant * (double)info.real_params[AC_dsmin] /
-          ((double)info.real_params[AC_cs_sound] + umax);
+    a.cfl_cdt = d->cfl_courant;
+    a.cfl_dsmin = (double)info.real_params[AC_dsmin];
+    a.cfl_cs = (double)info.real_params[AC_cs_sound];
+    set_cfl_args(d, a);
+    *dt = cfl_dt(a.cfl_mode, d->host_pinned[0], d->host_pinned[1], a.cfl_cdt, a.cfl_dsmin,
+                 a.cfl_cs, a.cfl_cs2, a.cfl_cdtv, a.cfl_visc);
     return AC_SUCCESS;
 }
 
@@ -857,13 +904,29 @@ AcResult acDeviceIntegrateSubstepDeviceDt(const Device d, const Stream stream,
         return AC_FAILURE;
     SolveArgs<AcReal> a;
     fill_solve_args(d, a, step_number, (AcReal)0);
-    a.cfl_cdt = (double)courant;
-    a.umax2_cur = (const double*)(d->umax2 + d->cfl_cur);
+    if (d->cfl_mode == 0)
+        a.cfl_cdt = (double)courant;
+    a.umax2_cur = (const double*)(d->umax2 + 2 * d->cfl_cur);
     const int nxt = 1 - d->cfl_cur;
     if (step_number == 2) {
-        launch_zero_bytes(d->umax2 + nxt, sizeof(unsigned long long), s);
-        a.umax2_next = d->umax2 + nxt;
+        unsigned long long* next = d->umax2 + 2 * nxt;
+        launch_zero_bytes(next, 2 * sizeof(unsigned long long), s);
         a.time_acc = d->time_acc;
+        if (d->cfl_mode == 0) {
+            a.umax2_next = next;  // max |u_new|^2, reduced by the solve kernel itself
+        }
+        else {
+            // The reference modes reduce |u| and vA of the state entering this substep: one
+            // pass over it right before the solve, on the same stream (fusing it into the solve
+            // would cost the solve kernels registers they need for two waves per SIMD).
+            Coeffs<AcReal> C;
+            make_coeffs(d->local_config, C);
+            const AcReal* f[NF];
+            for (int q = 0; q < NF; ++q)
+                f[q] = d->in[q];
+            const int3 n = d->local_config.int3_params[AC_nlocal];
+            launch_cfl_state<AcReal>(f, C, n.x, n.y, n.z, d->cfl_mode, a.cfl_cs2, next, s);
+        }
     }
     const AcResult r = launch_solve(d, a, s);
     if (step_number == 2)

@@ -87,6 +87,11 @@ struct FourierArgs {
 template <typename T>
 void launch_fourier_ic(const FourierArgs<T>& A, hipStream_t s);
 
+// ---- inputs of the reference timestep (cfl_cell modes 1, 2) for the current state ----
+template <typename T>
+void launch_cfl_state(const T* const* fields, const Coeffs<T>& C, int nx, int ny, int nz, int mode,
+                      double cs2, unsigned long long* out, hipStream_t s);
+
 // ---- one-pass diagnostics: per field min, max, sum of squares; then sum, max and min of
 // |u|^2, |curl A|^2 and |curl A|^2 / exp(lnrho)
 constexpr int DIAG_SLOTS = 3 * NF + 9;

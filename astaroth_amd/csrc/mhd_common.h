@@ -55,11 +55,15 @@ struct SolveArgs {
     T dt;              // used when dt_dev == nullptr
     T alpha, beta;     // alpha_s, beta_s
     int step;          // substep 0, 1, 2
-    // Device-resident CFL timestep (nullptr = use dt): dt = cdt * dsmin / (cs + sqrt(umax2[0]))
+    // Device-resident timestep (umax2_cur == nullptr: use dt). umax2_cur / umax2_next hold a
+    // pair (x, y) of reduced maxima (double bits); what they hold and the dt formula depend on
+    // cfl_mode (see cfl_cell / solve_dt).
     const double* umax2_cur;
-    unsigned long long* umax2_next;  // step 2: atomic max of |u_new|^2 (double bits)
+    unsigned long long* umax2_next;  // step 2: atomic max of the pair of the state reduced
     double* time_acc;                // step 2: accumulated simulated time
     double cfl_cdt, cfl_dsmin, cfl_cs;
+    int cfl_mode;                    // 0 courant, 1 reference, 2 reference additive
+    double cfl_cs2, cfl_cdtv, cfl_visc;  // cs^2, cdtv, max(nu, eta[, gamma chi])
 };
 
 __device__ __forceinline__ float ac_exp(float x) { return expf(x); }
@@ -77,6 +81,47 @@ struct CellDerivs {
     T ga[3][3], lapa[3], gda[3];
     T s, gs[3], laps;
 };
+
+// Timestep modes. 0: dt = cdt dsmin / (cs + max|u|), with max|u| of the new state, reduced by
+// the third substep's solve kernel (x = |u|^2).
+// 1: the reference's default CFL (ref samples/standalone_mpi/main.cc:395-440):
+//    dt = min(cdt dsmin / (max|u| + sqrt(cs^2 + max vA^2)), cdtv dsmin^2 / max(nu, eta)),
+//    x = |u|^2, y = vA^2 = |curl A|^2 / rho of the state entering the third substep, where the
+//    reference reduces them (ref mhdsolver.ac:551-583,611).
+// 2: the reference's additive CFL (AC_additive_timestep): dt = min over cells of
+//    cdt dsmin / (|u| + sqrt(cs^2 + vA^2)) and cdtv dsmin^2 / max(nu, eta, gamma chi),
+//    x = (|u| + sqrt(cs^2 + vA^2))^2.
+// cfl_cell gives (x, y) of one cell for modes 1 and 2 from its derivative quantities.
+template <typename T>
+__device__ __forceinline__ void cfl_cell(int mode, double cs2, const CellDerivs<T>& d, double& x,
+                                         double& y)
+{
+    const double u2 = (double)d.u[0] * d.u[0] + (double)d.u[1] * d.u[1] + (double)d.u[2] * d.u[2];
+    const double b0 = (double)d.ga[2][1] - d.ga[1][2];
+    const double b1 = (double)d.ga[0][2] - d.ga[2][0];
+    const double b2 = (double)d.ga[1][0] - d.ga[0][1];
+    const double va2 = (b0 * b0 + b1 * b1 + b2 * b2) / exp((double)d.lr);
+    if (mode == 1) {
+        x = u2;
+        y = va2;
+    }
+    else {
+        const double sm = sqrt(u2) + sqrt(cs2 + va2);
+        x = sm * sm;
+        y = 0.0;
+    }
+}
+
+__host__ __device__ __forceinline__ double cfl_dt(int mode, double x, double y, double cdt,
+                                                  double dsmin, double cs, double cs2, double cdtv,
+                                                  double visc)
+{
+    if (mode == 0)
+        return cdt * dsmin / (cs + sqrt(x));
+    const double adv = mode == 1 ? cdt * dsmin / (sqrt(x) + sqrt(cs2 + y)) : cdt * dsmin / sqrt(x);
+    const double dif = visc > 0.0 ? cdtv * dsmin * dsmin / visc : adv;
+    return adv < dif ? adv : dif;
+}
 
 // Right-hand side of the eight evolved fields at one cell.
 template <typename T, bool UPWD, bool ENTROPY>
@@ -360,8 +405,8 @@ __device__ __forceinline__ T solve_dt(const SolveArgs<T>& a)
 {
     if (a.umax2_cur == nullptr)
         return a.dt;
-    const double umax = sqrt(*a.umax2_cur);
-    return (T)(a.cfl_cdt * a.cfl_dsmin / (a.cfl_cs + umax));
+    return (T)cfl_dt(a.cfl_mode, a.umax2_cur[0], a.umax2_cur[1], a.cfl_cdt, a.cfl_dsmin, a.cfl_cs,
+                     a.cfl_cs2, a.cfl_cdtv, a.cfl_visc);
 }
 
 // Block-wide max of |u|^2 into umax2_next (step 2 of the device-dt path), plus time.

@@ -39,7 +39,7 @@ __global__ void __launch_bounds__(256) solve_direct_kernel(SolveArgs<T> a)
     const int j = threadIdx.y + blockIdx.y * blockDim.y;
     const int k = blockIdx.z;
     const bool active = i < a.nx && j < a.ny && k < a.nz;
-    double u2 = 0.0;
+    double u2 = 0.0;  // max |u_new|^2 (the courant timestep mode)
     const T dt = solve_dt(a);
     if (active) {
         const long long sx = 1, sy = a.mx, sz = a.sz;
@@ -126,8 +126,9 @@ __global__ void __launch_bounds__(256) solve_direct_kernel(SolveArgs<T> a)
         u2 = (double)fnew[1] * (double)fnew[1] + (double)fnew[2] * (double)fnew[2] +
              (double)fnew[3] * (double)fnew[3];
     }
-    if (a.step == 2 && a.umax2_next != nullptr) {
-        wave_max_to_global(u2, a.umax2_next);
+    if (a.step == 2) {
+        if (a.umax2_next != nullptr)
+            wave_max_to_global(u2, a.umax2_next);
         if (a.time_acc != nullptr && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 &&
             threadIdx.x == 0 && threadIdx.y == 0)
             *a.time_acc += (double)dt;

@@ -137,7 +137,7 @@ __global__ void __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY == 256 ? 2 
             ch[c][i] = T(0);
 
     const T dt = solve_dt(a);
-    double u2max = 0.0;
+    double u2max = 0.0;  // max |u_new|^2 (the courant timestep mode)
     const int gx = x0 + tx, gy = y0 + ty;
     const int ctr = (ty + R) * LX + (tx + R);
     const unsigned ioff = (unsigned)((gx + R) + (int)sy * (gy + R));  // in-plane offset
@@ -411,8 +411,9 @@ __global__ void __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY == 256 ? 2 
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
     }
-    if (a.step == 2 && a.umax2_next != nullptr) {
-        wave_max_to_global(u2max, a.umax2_next);
+    if (a.step == 2) {
+        if (a.umax2_next != nullptr)
+            wave_max_to_global(u2max, a.umax2_next);
         if (a.time_acc != nullptr && blockIdx.x == 0 && tid == 0)
             *a.time_acc += (double)dt;
     }

@@ -376,9 +376,21 @@ AcResult acDeviceReduceVecScal(const Device device, const Stream stream,
 
 /* ---- Extensions with no one-call counterpart in the reference ---- */
 
-/* Device-resident CFL timestep: dt = courant * dsmin / (cs_sound + max|u|), with max|u|
- * reduced on the device in FP64. Replaces the host round trip of
- * ref samples/standalone_mpi/main.cc:395-440 (which reads UU_MAX_ADVEC back each step). */
+/* Timestep formula of the device-resident dt below.
+ * AC_TIMESTEP_COURANT (default): dt = courant * dsmin / (cs_sound + max|u|), max|u| of the state
+ *   produced by the last step.
+ * AC_TIMESTEP_REFERENCE: the reference's CFL (ref samples/standalone_mpi/main.cc:395-440 and
+ *   acc-runtime/samples/mhd_modular/mhdsolver.ac:551-583): with AC_additive_timestep false,
+ *   dt = min(AC_cdt dsmin / (max|u| + sqrt(cs^2 + max vA^2)), AC_cdtv dsmin^2 / max(nu, eta));
+ *   with it true, dt = min over cells of AC_cdt dsmin / (|u| + sqrt(cs^2 + vA^2)) and
+ *   AC_cdtv dsmin^2 / max(nu, eta, gamma chi); vA = |curl A| / sqrt(exp(lnrho)). The maxima are
+ *   reduced over the state entering the third substep, inside that substep, as the reference
+ *   does; the courant arguments below are then ignored. */
+enum { AC_TIMESTEP_COURANT = 0, AC_TIMESTEP_REFERENCE = 1 };
+AcResult acDeviceSetTimestepMode(const Device device, const int mode);
+/* Device-resident CFL timestep, reduced on the device in FP64 from the current state. Replaces
+ * the host round trip of ref samples/standalone_mpi/main.cc:395-440 (which reads UU_MAX_ADVEC
+ * back each step). */
 AcResult acDeviceUpdateTimestepCFL(const Device device, const Stream stream, const AcReal courant);
 /* Returns the current device-resident dt (synchronises the stream). */
 AcResult acDeviceGetTimestep(const Device device, const Stream stream, double* dt);

@@ -357,3 +357,64 @@ def test_diagnostics_magnetic_and_alfven(lib):
                             ("uu_min", d.uu_min, np.sqrt(u2.min()))):
         say(f"  {name}: {got:.9g} vs {want:.9g}")
         assert got == pytest.approx(want, rel=tol, abs=tol * np.sqrt(b2.max())), name
+
+
+def reference_dt_numpy(data, info, additive):
+    """The reference's CFL (ref samples/standalone_mpi/main.cc:395-440, mhdsolver.ac:551-583)
+    on interior cells of a halo-filled state, in float64."""
+    ds = [float(v) for v in info["AC_ds"]]
+    dsmin = min(ds)
+    bx, by, bz = curl_6th(data[4:7], ds)
+    sl = O.interior_slices(data.shape[1:])
+    va2 = (bx * bx + by * by + bz * bz) / np.exp(data[0][sl].astype(np.float64))
+    u2 = sum(data[q][sl].astype(np.float64) ** 2 for q in (1, 2, 3))
+    cs2 = float(info["AC_cs_sound"]) ** 2
+    cdt, cdtv = float(info["AC_cdt"]), float(info["AC_cdtv"])
+    nu, eta = float(info["AC_nu_visc"]), float(info["AC_eta"])
+    if additive:
+        visc = max(nu, eta, float(info["AC_gamma"]) * float(info["AC_chi"]))
+        adv = cdt * dsmin / np.max(np.sqrt(u2) + np.sqrt(cs2 + va2))
+    else:
+        visc = max(nu, eta)
+        adv = cdt * dsmin / (np.sqrt(u2.max()) + np.sqrt(cs2 + va2.max()))
+    return min(adv, cdtv * dsmin * dsmin / visc)
+
+
+@pytest.mark.parametrize("additive", [False, True])
+def test_reference_timestep(lib, additive):
+    say(f"case reference timestep {lib.precision} additive={additive}")
+    info = lib.config((24, 20, 16), AC_len=(2 * np.pi,) * 3, AC_cs_sound=1.0, AC_gamma=5.0 / 3.0,
+                      AC_nu_visc=5e-3, AC_eta=5e-3, AC_chi=5e-3, AC_K_heatcond=5e-3, AC_cdt=0.4,
+                      AC_cdtv=0.3, AC_additive_timestep=additive)
+    data = c1_initial(lib, info, seed=21)
+    data[4:7] *= 6.0  # a field strong enough that the Alfven speed matters
+    data[0] += 0.05 * np.cos(data[1] * 9.0).astype(data.dtype)
+    O.periodic_fill(data, O.config_from_info(info))
+    tol = 1e-5 if lib.precision == "fp32" else 1e-12
+    mesh = ac.HostMesh(lib, info, data)
+    with ac.Device(lib, info) as dev, ac.Device(lib, info) as ref:
+        for dv in (dev, ref):
+            dv.load_mesh(mesh)
+            dv.periodic()
+            ac.check(lib.acDeviceSetTimestepMode(dv.handle, ac.AC_TIMESTEP_REFERENCE), "mode")
+        ac.check(lib.acDeviceUpdateTimestepCFL(dev.handle, 0, lib.np_real(0.2)), "cfl")
+        dt0 = ctypes.c_double()
+        ac.check(lib.acDeviceGetTimestep(dev.handle, 0, ctypes.byref(dt0)), "dt")
+        want0 = reference_dt_numpy(data, info, additive)
+        say(f"  initial dt {dt0.value:.12g} vs {want0:.12g}")
+        assert dt0.value == pytest.approx(want0, rel=tol)
+        # one step with the device timestep; the next dt comes from the state entering substep 2
+        ac.check(lib.acDeviceIntegrateStepsCFL(dev.handle, 0, 1, lib.np_real(0.2), None), "step")
+        dt1 = ctypes.c_double()
+        ac.check(lib.acDeviceGetTimestep(dev.handle, 0, ctypes.byref(dt1)), "dt")
+        for s in (0, 1):
+            ref.periodic()
+            ref.substep(s, lib.np_real(dt0.value))
+            ref.swap()
+        ref.periodic()
+        mid = ac.HostMesh(lib, info)
+        ref.store_mesh(mid)
+        want1 = reference_dt_numpy(mid.data, info, additive)
+        say(f"  dt after one step {dt1.value:.12g} vs {want1:.12g}")
+        assert dt1.value == pytest.approx(want1, rel=tol)
+        assert dt1.value != dt0.value
