@@ -91,6 +91,8 @@ EXPORTED = [
     "acAdjointWorkspaceBytes", "acDeviceIntegrateSubstepTangent", "acDeviceIntegrateSubstepAdjoint",
     "acDeviceLoadMeshFromDevicePtr", "acDeviceStoreMeshToDevicePtr", "acDeviceInitFourier",
     "acHostAppendTimeseries", "acGetVertexBufferName", "acDeviceSetTimestepMode",
+    "acDeviceWriteSnapshot", "acDeviceReadSnapshot", "acHostAppendSnapshotInfo",
+    "acHostReadSnapshotInfo",
 ]
 
 
@@ -285,6 +287,10 @@ class AstarothLib:
             "acDeviceInitFourier": [dev, i, ctypes.c_ulonglong, i, i, ctypes.c_double,
                                     ctypes.c_double],
             "acDeviceSetTimestepMode": [dev, i],
+            "acDeviceWriteSnapshot": [dev, i, ctypes.c_char_p, ctypes.c_char_p],
+            "acDeviceReadSnapshot": [dev, i, ctypes.c_char_p, ctypes.c_char_p],
+            "acHostAppendSnapshotInfo": [ctypes.c_char_p, i, i, ctypes.c_double, P(Info)],
+            "acHostReadSnapshotInfo": [ctypes.c_char_p, P(i), P(i), P(ctypes.c_double)],
         }
         sig["acHostAppendTimeseries"] = [ctypes.c_char_p, i, ctypes.c_double, ctypes.c_double,
                                          ctypes.POINTER(AcDiagnostics), ctypes.POINTER(ctypes.c_int)]

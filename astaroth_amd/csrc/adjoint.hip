@@ -339,9 +339,12 @@ __global__ void __launch_bounds__(256) adjoint_transpose_z_kernel(AdjArgs<T> A, 
         const long long nb = opaque_s64(A.count * (long long)sizeof(T));
         const unsigned colb = opaque(colb0);
         const char* const pb = (const char*)A.g + sz * (long long)(p + R) * (long long)sizeof(T);
-        // value of derivative-adjoint field k at offset (dx, dy) from this column in plane p
+        // value of derivative-adjoint field k at offset (dx, dy) from this column in plane p;
+        // the signed neighbour offset is added to the column's byte offset in 32-bit unsigned
+        // arithmetic first (the sum is a valid non-negative offset), then to the base pointer
         auto ld = [&](int k, int dx, int dy) -> T {
-            return *(const T*)(pb + k * nb + colb + (unsigned)((dx + dy * sy) * (int)sizeof(T)));
+            const unsigned off = colb + (unsigned)((dx + dy * sy) * (int)sizeof(T));
+            return *(const T*)(pb + k * nb + off);
         };
         auto d1x = [&](int k) {
             return C.d1[0][1] * (ld(k, 1, 0) - ld(k, -1, 0)) + C.d1[0][2] * (ld(k, 2, 0) - ld(k, -2, 0)) +

@@ -1218,3 +1218,107 @@ extern "C" AcResult acDeviceStoreMeshToDevicePtr(const Device d, const Stream st
                                     hipMemcpyDeviceToDevice, s));
     return AC_SUCCESS;
 }
+
+// ------------------------------------------------------------------ snapshots on disk
+// The reference's snapshot format (ref src/core/grid.cc:4405-4530 with collective IO, and
+// samples/standalone_mpi/main.cc:155-216): one raw file per evolved field,
+// "<dir>/<field name>-<label>.mesh", holding the interior cells only (nx * ny * nz AcReal values,
+// x fastest), plus the restart table snapshots_info.csv written by acHostAppendSnapshotInfo.
+extern "C" const char* acGetVertexBufferName(const VertexBufferHandle handle);
+
+static std::string snapshot_path(const char* dir, int q, const char* label)
+{
+    return std::string(dir) + "/" + acGetVertexBufferName((VertexBufferHandle)q) + "-" + label +
+           ".mesh";
+}
+
+extern "C" AcResult acDeviceWriteSnapshot(const Device d, const Stream stream, const char* dir,
+                                          const char* label)
+{
+    if (!dir || !label)
+        return fail("acDeviceWriteSnapshot: dir and label are required");
+    AC_HIP_CHECK(hipSetDevice(d->id));
+    hipStream_t s = get_stream(d, stream);
+    const int3 n = d->local_config.int3_params[AC_nlocal];
+    const size_t mx = (size_t)n.x + 2 * R, my = (size_t)n.y + 2 * R;
+    const size_t plane = (size_t)n.x * n.y;
+    AcReal* host = nullptr;
+    AC_HIP_CHECK(hipHostMalloc((void**)&host, plane * n.z * sizeof(AcReal)));
+    AcResult r = AC_SUCCESS;
+    for (int q = 0; q < NF && r == AC_SUCCESS; ++q) {
+        for (int k = 0; k < n.z; ++k) {
+            const AcReal* src = d->in[q] + R + mx * (R + my * (size_t)(k + R));
+            if (hipMemcpy2DAsync(host + plane * k, n.x * sizeof(AcReal), src, mx * sizeof(AcReal),
+                                 n.x * sizeof(AcReal), n.y, hipMemcpyDeviceToHost, s) != hipSuccess) {
+                r = fail("acDeviceWriteSnapshot: device-to-host copy failed");
+                break;
+            }
+        }
+        if (r != AC_SUCCESS || hipStreamSynchronize(s) != hipSuccess) {
+            r = AC_FAILURE;
+            break;
+        }
+        const std::string path = snapshot_path(dir, q, label);
+        FILE* fp = fopen(path.c_str(), "wb");
+        if (!fp) {
+            r = fail("acDeviceWriteSnapshot: cannot open a snapshot file for writing");
+            break;
+        }
+        const size_t cnt = plane * n.z;
+        if (fwrite(host, sizeof(AcReal), cnt, fp) != cnt)
+            r = fail("acDeviceWriteSnapshot: short write");
+        fclose(fp);
+    }
+    hipHostFree(host);
+    return r;
+}
+
+extern "C" AcResult acDeviceReadSnapshot(const Device d, const Stream stream, const char* dir,
+                                         const char* label)
+{
+    if (!dir || !label)
+        return fail("acDeviceReadSnapshot: dir and label are required");
+    AC_HIP_CHECK(hipSetDevice(d->id));
+    hipStream_t s = get_stream(d, stream);
+    const int3 n = d->local_config.int3_params[AC_nlocal];
+    const size_t mx = (size_t)n.x + 2 * R, my = (size_t)n.y + 2 * R;
+    const size_t plane = (size_t)n.x * n.y;
+    const size_t cnt = plane * n.z;
+    AcReal* host = nullptr;
+    AC_HIP_CHECK(hipHostMalloc((void**)&host, cnt * sizeof(AcReal)));
+    AcResult r = AC_SUCCESS;
+    for (int q = 0; q < NF && r == AC_SUCCESS; ++q) {
+        const std::string path = snapshot_path(dir, q, label);
+        FILE* fp = fopen(path.c_str(), "rb");
+        if (!fp) {
+            r = fail("acDeviceReadSnapshot: cannot open a snapshot file");
+            break;
+        }
+        const size_t got = fread(host, sizeof(AcReal), cnt, fp);
+        const bool extra = fgetc(fp) != EOF;
+        fclose(fp);
+        if (got != cnt || extra) {
+            r = fail("acDeviceReadSnapshot: snapshot file size does not match the grid");
+            break;
+        }
+        for (int k = 0; k < n.z; ++k) {
+            AcReal* dst = d->in[q] + R + mx * (R + my * (size_t)(k + R));
+            if (hipMemcpy2DAsync(dst, mx * sizeof(AcReal), host + plane * k, n.x * sizeof(AcReal),
+                                 n.x * sizeof(AcReal), n.y, hipMemcpyHostToDevice, s) != hipSuccess) {
+                r = fail("acDeviceReadSnapshot: host-to-device copy failed");
+                break;
+            }
+        }
+        // the pinned buffer is reused for the next field: wait for this one's copies
+        if (hipStreamSynchronize(s) != hipSuccess)
+            r = AC_FAILURE;
+    }
+    hipHostFree(host);
+    if (r != AC_SUCCESS)
+        return r;
+    launch_periodic_fill<AcReal>(d->in, NF, n.x, n.y, n.z, s);
+    if (d->kahan_bits > 0)
+        launch_zero_bytes(d->slab_comp, d->count * (size_t)(d->kahan_bits / 8) * NF, s);
+    AC_HIP_CHECK(hipGetLastError());
+    return AC_SUCCESS;
+}

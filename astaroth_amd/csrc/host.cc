@@ -601,6 +601,70 @@ AcResult acHostAppendTimeseries(const char* path, const int step, const double t
     return ok ? AC_SUCCESS : AC_FAILURE;
 }
 
+// The restart table of the reference's standalone driver (ref
+// samples/standalone_mpi/main.cc:186-203): header at step 0 (the file is then truncated), one
+// row per snapshot "use_double,mx,my,mz,step_number,modstep,t_step,t_step(exact)" with the
+// time printed as %g and as a hexadecimal float (%a), which reads back exactly.
+AcResult acHostAppendSnapshotInfo(const char* dir, const int step, const int modstep,
+                                  const double t_step, const AcMeshInfo* info)
+{
+    if (!dir || !info)
+        return AC_FAILURE;
+    const std::string path = std::string(dir) + "/snapshots_info.csv";
+    FILE* fp = std::fopen(path.c_str(), step == 0 ? "w" : "a");
+    if (!fp)
+        return AC_FAILURE;
+    if (step == 0)
+        std::fprintf(fp, "use_double,mx,my,mz,step_number,modstep,t_step,t_step(exact)\n");
+    const int3 m = info->int3_params[AC_mlocal];
+    std::fprintf(fp, "%d,%d,%d,%d,%d,%d,%g,%a\n", (int)(sizeof(AcReal) == 8), m.x, m.y, m.z, step,
+                 modstep, t_step, t_step);
+    const bool ok = std::fflush(fp) == 0;
+    std::fclose(fp);
+    return ok ? AC_SUCCESS : AC_FAILURE;
+}
+
+// Last row of snapshots_info.csv: the snapshot a restart loads (ref main.cc:579-605).
+AcResult acHostReadSnapshotInfo(const char* dir, int* step, int* modstep, double* t_step)
+{
+    if (!dir)
+        return AC_FAILURE;
+    const std::string path = std::string(dir) + "/snapshots_info.csv";
+    FILE* fp = std::fopen(path.c_str(), "r");
+    if (!fp)
+        return AC_FAILURE;
+    char line[1024];
+    bool header = true, found = false;
+    int s = 0, ms = 0;
+    double t = 0.0;
+    while (std::fgets(line, sizeof(line), fp)) {
+        if (header) {
+            header = false;
+            continue;
+        }
+        int ud, mx, my, mz, s1, ms1;
+        double tg;
+        char hex[256];
+        if (std::sscanf(line, "%d,%d,%d,%d,%d,%d,%lg,%255s", &ud, &mx, &my, &mz, &s1, &ms1, &tg,
+                        hex) == 8) {
+            s = s1;
+            ms = ms1;
+            t = std::strtod(hex, nullptr);
+            found = true;
+        }
+    }
+    std::fclose(fp);
+    if (!found)
+        return AC_FAILURE;
+    if (step)
+        *step = s;
+    if (modstep)
+        *modstep = ms;
+    if (t_step)
+        *t_step = t;
+    return AC_SUCCESS;
+}
+
 AcResult acHostMeshInitFourier(AcMesh* mesh, const unsigned long long seed, const int nmodes,
                                const int kmax, const double ampl, const double lnrho_perturb)
 {

@@ -429,6 +429,16 @@ AcResult acDeviceGetStreamHandle(const Device device, const Stream stream, void*
 AcResult acDeviceLoadMeshFromDevicePtr(const Device device, const Stream stream, const AcReal* src);
 AcResult acDeviceStoreMeshToDevicePtr(const Device device, const Stream stream, AcReal* dst);
 
+/* Snapshots in the reference's on-disk format (ref src/core/grid.cc:4405-4530, collective IO on
+ * one device): per evolved field one raw file dir/<field name>-<label>.mesh of the interior
+ * cells (nx*ny*nz AcReal, x fastest). Write stores the current state (in); read loads it into
+ * the current state, fills the halos periodically and clears the compensation; a file whose
+ * size does not match the grid is an error. Both synchronise the stream. */
+AcResult acDeviceWriteSnapshot(const Device device, const Stream stream, const char* dir,
+                               const char* label);
+AcResult acDeviceReadSnapshot(const Device device, const Stream stream, const char* dir,
+                              const char* label);
+
 /* The seeded Fourier-mode initial condition of acHostMeshInitFourier, generated directly in the
  * device's input buffers (bit-identical to the host version followed by acDeviceLoadMesh), with
  * halos filled and the output and compensation buffers cleared. For grids whose host mesh is
@@ -496,6 +506,14 @@ const char* acGetVertexBufferName(const VertexBufferHandle handle);
  * reduction is NaN, the condition on which that driver stops. */
 AcResult acHostAppendTimeseries(const char* path, const int step, const double t_step,
                                 const double dt, const AcDiagnostics* diag, int* found_nan);
+
+/* Snapshot restart table in the reference's format (ref samples/standalone_mpi/main.cc:186-203):
+ * dir/snapshots_info.csv, truncated with a header at step 0, then one row per snapshot
+ * "use_double,mx,my,mz,step_number,modstep,t_step,t_step(exact)". */
+AcResult acHostAppendSnapshotInfo(const char* dir, const int step, const int modstep,
+                                  const double t_step, const AcMeshInfo* info);
+/* The last row of dir/snapshots_info.csv (the snapshot a restart loads, ref main.cc:579-605). */
+AcResult acHostReadSnapshotInfo(const char* dir, int* step, int* modstep, double* t_step);
 
 /* Library identification: sizeof(AcReal), sizeof(AcMeshInfo), sizeof(AcMesh) of this build and
  * the offload target it was built for (a caller checks its own struct sizes against these, as
