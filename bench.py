@@ -250,7 +250,8 @@ def fixed_steps(dev, dt, steps):
     return time.perf_counter() - t0
 
 
-def form_precision(ac, n=512, steps=200, mid=100, seed=512, kbs=(8, 16, 32)):
+def form_precision(ac, n=512, steps=200, mid=100, seed=512, kbs=(8, 16, 32), nmodes=64, kmax=8,
+                   ampl=0.1, perturb=1e-4, with_va=True):
     """512^3, 64 modes |k| <= 8, amplitude 0.1, lnrho + U(-1e-4, 1e-4); FP64 reference, plain FP32
     and Kahan-compensated FP32 (compensation stored in 8, 16 or 32 bits). This initial condition
     is magnetically dominated (Alfven speed well above cs + |u|), so dt is recomputed every step
@@ -262,14 +263,14 @@ def form_precision(ac, n=512, steps=200, mid=100, seed=512, kbs=(8, 16, 32)):
     l64 = ac.load("fp64")
     info64 = l64.config((n, n, n), **PHYS)
     dev = ac.Device(l64, info64)
-    dev.init_fourier(seed, 64, 8, 0.1, 1e-4)
+    dev.init_fourier(seed, nmodes, kmax, ampl, perturb)
     dts, vmax = [], []
     t_ref = 0.0
     refs = {}
     for k in range(steps):
         dev.periodic()
         d = dev.diagnostics()
-        dt = 0.2 * dx / (1.0 + d.uu_max + d.va_max)
+        dt = 0.2 * dx / (1.0 + d.uu_max + (d.va_max if with_va else 0.0))
         dts.append(dt)
         vmax.append((d.uu_max, d.va_max))
         dev.sync()
@@ -284,7 +285,7 @@ def form_precision(ac, n=512, steps=200, mid=100, seed=512, kbs=(8, 16, 32)):
     for kb in (0,) + tuple(kbs):
         l32 = ac.load("fp32")
         dv = ac.Device(l32, l32.config((n, n, n), AC_kahan_bits=kb, **PHYS))
-        dv.init_fourier(seed, 64, 8, 0.1, 1e-4)
+        dv.init_fourier(seed, nmodes, kmax, ampl, perturb)
         t_run = 0.0
         errs = {}
         for k in range(steps):
@@ -305,7 +306,9 @@ def form_precision(ac, n=512, steps=200, mid=100, seed=512, kbs=(8, 16, 32)):
     return {"form": "precision", "grid": [n, n, n], "steps": steps,
             "dt_first_last": [dts[0], dts[-1]], "max_u_first_last": [vmax[0][0], vmax[-1][0]],
             "max_vA_first_last": [vmax[0][1], vmax[-1][1]],
-            "ic": "64 modes |k|<=8 ampl 0.1, lnrho U(-1e-4,1e-4), seed %d" % seed,
+            "ic": f"{nmodes} modes |k|<={kmax} ampl {ampl}, lnrho U(-{perturb},{perturb}), seed {seed}",
+            "dt_rule": "0.2 dx / (cs + max|u|" + (" + max vA)" if with_va else ")") +
+                       " of the FP64 state, every step, replayed in the FP32 runs",
             "timing": "wall time of the steps only, device-synchronised per step",
             "reference": "FP64 GPU mode", "runs": runs}
 
@@ -345,8 +348,11 @@ def form_scale_adjoint(ac, n=384, steps=500, snaps=24, seed=384):
     runner = AD.MHDAdjoint(lib, info, steps, 1.0, schedule="revolve", snaps=snaps)
     try:
         runner.dev.init_fourier(seed, 256, 16, 0.1, 0.0)
-        umax = runner.dev.reduce_vec("RTYPE_MAX", 1, 2, 3)
-        runner.dt = lib.np_real(0.2 * (2 * np.pi / n) / (1.0 + umax))
+        runner.dev.periodic()
+        d = runner.dev.diagnostics()
+        # fixed dt for the whole run: the field is strong (|k| up to 16), so the Alfven speed
+        # enters the CFL bound; a fifth of it leaves room for the flow to speed up
+        runner.dt = lib.np_real(0.2 * 0.2 * (2 * np.pi / n) / (1.0 + d.uu_max + d.va_max))
         f0 = state_tensor(lib, runner.dev)
         runner.dev.init_fourier(seed + 7, 256, 16, 0.1, 0.0)
         tgt = state_tensor(lib, runner.dev)[1:4].clone()
@@ -370,7 +376,8 @@ def form_scale_adjoint(ac, n=384, steps=500, snaps=24, seed=384):
                 "forward_s": t_fwd, "reverse_s": t_bwd,
                 "reverse_over_forward": t_bwd / t_fwd,
                 "recomputed_steps": rec, "recomputed_per_step": rec / steps,
-                "loss": float(loss), "grad_nu": float(nu.grad), "grad_eta": float(eta.grad)}
+                "dt": float(runner.dt), "loss": float(loss.detach()),
+                "grad_nu": float(nu.grad), "grad_eta": float(eta.grad)}
     finally:
         runner.close()
 
@@ -466,7 +473,8 @@ def main():
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--variant-steps", type=int, default=30)
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--form", choices=["precision", "scale", "scale_adjoint", "differentiation"],
+    ap.add_argument("--form", choices=["precision", "precision_plain", "scale", "scale_adjoint",
+                                       "differentiation"],
                     help="run one measurement form instead of the headline benchmark")
     ap.add_argument("--shape", type=int, nargs=3, default=(0, 0, 0),
                     help="pin the solve launch shape TX TY ZC (0 0 0 = autotune)")
@@ -482,7 +490,12 @@ def main():
     from astaroth_amd import astaroth as ac
     if args.form:
         fn = {"precision": form_precision, "scale": form_scale,
-              "scale_adjoint": form_scale_adjoint, "differentiation": form_differentiation}
+              "scale_adjoint": form_scale_adjoint, "differentiation": form_differentiation,
+              # the plain-method workload (256^3, 16 modes |k| <= 4, amplitude 0.05, 100 steps,
+              # dt = 0.2 dx / (cs + max|u|)) in the three precisions
+              "precision_plain": lambda ac_: form_precision(ac_, n=256, steps=100, mid=50,
+                                                            seed=2024, nmodes=16, kmax=4,
+                                                            ampl=0.05, perturb=0.0, with_va=False)}
         print(json.dumps(fn[args.form](ac)), flush=True)
         return
     n = args.n
