@@ -18,14 +18,6 @@
 
 namespace acamd {
 
-constexpr int NG = 55;  // derivative-adjoint fields
-// layout of the derivative adjoints
-constexpr int G_LR = 0;                       // val, D1 x y z, lap, up x y z
-constexpr int G_U = 8;                        // + 8 i: val, D1 x y z, lap, up x y z
-constexpr int G_GDU = 32;                     // grad(div u) x y z
-constexpr int G_A = 35;                       // + 4 i: D1 x y z, lap
-constexpr int G_GDA = 47;                     // grad(div A) x y z
-constexpr int G_S = 50;                       // val, D1 x y z, lap
 
 template <typename T>
 struct AdjArgs {
@@ -191,31 +183,7 @@ __global__ void __launch_bounds__(256) adjoint_local_kernel(AdjArgs<T> A)
         peta = (double)a.dt * (double)aeta;
         T* g = A.g;
         const long long n = A.count;
-        auto put = [&](int k, T v) { g[(long long)k * n + c] = v; };
-        put(G_LR + 0, ad.lr);
-#pragma unroll
-        for (int ax = 0; ax < 3; ++ax) {
-            put(G_LR + 1 + ax, ad.glr[ax]);
-            put(G_LR + 5 + ax, aup.ul[ax]);
-        }
-        put(G_LR + 4, ad.laplr);
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            put(G_U + 8 * i + 0, ad.u[i]);
-#pragma unroll
-            for (int ax = 0; ax < 3; ++ax) {
-                put(G_U + 8 * i + 1 + ax, ad.gu[i][ax]);
-                put(G_U + 8 * i + 5 + ax, aup.uu[i][ax]);
-                put(G_A + 4 * i + ax, ad.ga[i][ax]);
-            }
-            put(G_U + 8 * i + 4, ad.lapu[i]);
-            put(G_A + 4 * i + 3, ad.lapa[i]);
-            put(G_GDU + i, ad.gdu[i]);
-            put(G_GDA + i, ad.gda[i]);
-            put(G_S + 1 + i, ad.gs[i]);
-        }
-        put(G_S + 0, ad.s);
-        put(G_S + 4, ad.laps);
+        put_derivative_adjoints(ad, aup, [&](int k, T v) { g[(long long)k * n + c] = v; });
     }
     // block sums of the parameter sensitivities (FP64)
 #pragma unroll
@@ -511,8 +479,14 @@ void launch_adjoint(const AdjArgs<T>& A, bool upwd, bool entropy, double* param_
         else if (entropy) hipLaunchKernelGGL((K<T, false, true>), bpg, tpb, 0, s, A);              \
         else hipLaunchKernelGGL((K<T, false, false>), bpg, tpb, 0, s, A);                          \
     } while (0)
-    AC_ADJ(adjoint_local_kernel);
-    const long long nb = (long long)bpg.x * bpg.y * bpg.z;
+    // (A) derivative adjoints: the tiled kernel where the grid fits its 64 x 4 tiles (it has at
+    // most as many blocks, so its partial sums fit the same workspace), else one cell per thread
+    long long nb = 0;
+    const AdjLocal<T> xl{A.g, A.lf, A.lw, A.partials};
+    if (!launch_adjoint_local_tiled<T>(A.s, xl, upwd, entropy, s, &nb)) {
+        AC_ADJ(adjoint_local_kernel);
+        nb = (long long)bpg.x * bpg.y * bpg.z;
+    }
     double* ranges = A.partials + 2 * nb;
     hipLaunchKernelGGL(adjoint_param_stage1, dim3(PARAM_RANGES), dim3(256), 0, s,
                        (const double*)A.partials, nb, ranges);

@@ -1129,32 +1129,34 @@ static AcResult autotune(Device d, hipStream_t s)
     AC_HIP_CHECK(hipMemcpyAsync(wsave, d->slab_w, wbytes, hipMemcpyDeviceToDevice, s));
     SolveArgs<AcReal> a;
     fill_solve_args(d, a, 1, (AcReal)1e-9);
-    float best_ms = 1e30f;
+    // Each candidate: one warm-up launch, then the best of three timed pairs of launches (the
+    // minimum is robust against another process's work or a clock ramp during one sample).
+    auto time_candidate = [&](auto&& launch) -> float {
+        launch();
+        float best = 1e30f;
+        for (int rep = 0; rep < 3; ++rep) {
+            if (hipEventRecord(e0, s) != hipSuccess)
+                return 1e30f;
+            launch();
+            launch();
+            if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
+                return 1e30f;
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+                return 1e30f;
+            best = std::min(best, ms);
+        }
+        return best;
+    };
     // direct kernel as the baseline candidate
-    {
-        launch_solve_direct<AcReal>(a, upwd, entropy, 0, s);
-        AC_HIP_CHECK(hipEventRecord(e0, s));
-        for (int r = 0; r < 2; ++r)
-            launch_solve_direct<AcReal>(a, upwd, entropy, 0, s);
-        AC_HIP_CHECK(hipEventRecord(e1, s));
-        AC_HIP_CHECK(hipEventSynchronize(e1));
-        float ms = 0.f;
-        AC_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-        best_ms = ms;
-    }
+    float best_ms = time_candidate([&] { launch_solve_direct<AcReal>(a, upwd, entropy, 0, s); });
     for (int i = 0; i < tiled_num_shapes(); ++i) {
         const TiledShape sh = tiled_shape(i);
         if (!tiled_shape_fits(sh, n.x, n.y, n.z))
             continue;
         if (!launch_solve_tiled<AcReal>(a, upwd, entropy, 0, sh, s))
             continue;
-        AC_HIP_CHECK(hipEventRecord(e0, s));
-        for (int r = 0; r < 2; ++r)
-            launch_solve_tiled<AcReal>(a, upwd, entropy, 0, sh, s);
-        AC_HIP_CHECK(hipEventRecord(e1, s));
-        AC_HIP_CHECK(hipEventSynchronize(e1));
-        float ms = 0.f;
-        AC_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        const float ms = time_candidate([&] { launch_solve_tiled<AcReal>(a, upwd, entropy, 0, sh, s); });
         if (ms < best_ms) {
             best_ms = ms;
             best = sh;

@@ -46,6 +46,20 @@ bool launch_solve_tiled(const SolveArgs<T>& a, bool upwd, bool entropy, int kaha
 template <typename T>
 bool launch_solve_ring(const SolveArgs<T>& a, bool upwd, bool entropy, int kahan_bits,
                        TiledShape shape, hipStream_t stream);
+// The adjoint's local stage on the tiled machinery: derivative adjoints g (NG fields of m^3,
+// same field stride as the state), lambda_f / lambda_w (NF fields), (nu, eta) block partials.
+template <typename T>
+struct AdjLocal {
+    T* g;
+    const T* lf;
+    const T* lw;
+    double* partials;
+};
+// Returns false (nothing launched) when the grid does not fit 64 x 4 tiles.
+template <typename T>
+bool launch_adjoint_local_tiled(const SolveArgs<T>& a, const AdjLocal<T>& x, bool upwd,
+                                bool entropy, hipStream_t s, long long* nblocks);
+
 // LDS pointer type of the global->LDS DMA builtin
 typedef __attribute__((address_space(3))) void* lds_ptr_t;
 

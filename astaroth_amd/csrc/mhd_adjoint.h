@@ -11,6 +11,16 @@
 
 namespace acamd {
 
+// Layout of the derivative adjoints (the adjoint of every derivative quantity the right-hand
+// side reads), one m^3 field each, NG in all.
+constexpr int NG = 55;
+constexpr int G_LR = 0;   // lnrho: val, D1 x y z, lap, up x y z
+constexpr int G_U = 8;    // + 8 i, u_i: val, D1 x y z, lap, up x y z
+constexpr int G_GDU = 32; // grad(div u) x y z
+constexpr int G_A = 35;   // + 4 i, A_i: D1 x y z, lap
+constexpr int G_GDA = 47; // grad(div A) x y z
+constexpr int G_S = 50;   // ss: val, D1 x y z, lap
+
 // Upwind derivatives per direction: ul[j] = up_j(lnrho), uu[i][j] = up_j(u_i).
 template <typename T>
 struct UpwindDirs {
@@ -410,6 +420,37 @@ __device__ __forceinline__ void mhd_rhs_vjp(const CellDerivs<T>& d, const Upwind
             a.u[k] += sg * acc;
         }
     }
+}
+
+// Hands the derivative adjoints of one cell to put(field index, value) in the layout above.
+template <typename T, typename Put>
+__device__ __forceinline__ void put_derivative_adjoints(const CellDerivs<T>& ad,
+                                                        const UpwindDirs<T>& aup, Put&& put)
+{
+    put(G_LR + 0, ad.lr);
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        put(G_LR + 1 + ax, ad.glr[ax]);
+        put(G_LR + 5 + ax, aup.ul[ax]);
+    }
+    put(G_LR + 4, ad.laplr);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        put(G_U + 8 * i + 0, ad.u[i]);
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            put(G_U + 8 * i + 1 + ax, ad.gu[i][ax]);
+            put(G_U + 8 * i + 5 + ax, aup.uu[i][ax]);
+            put(G_A + 4 * i + ax, ad.ga[i][ax]);
+        }
+        put(G_U + 8 * i + 4, ad.lapu[i]);
+        put(G_A + 4 * i + 3, ad.lapa[i]);
+        put(G_GDU + i, ad.gdu[i]);
+        put(G_GDA + i, ad.gda[i]);
+        put(G_S + 1 + i, ad.gs[i]);
+    }
+    put(G_S + 0, ad.s);
+    put(G_S + 4, ad.laps);
 }
 
 }  // namespace acamd

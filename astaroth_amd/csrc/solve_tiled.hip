@@ -22,6 +22,7 @@
 //  * Workgroups are numbered so that consecutive tiles (which share halo rows) land on the same
 //    XCD and its L2: hardware dispatches workgroup b to XCD b % 8.
 #include "kernels.h"
+#include "mhd_adjoint.h"
 
 namespace acamd {
 
@@ -33,6 +34,7 @@ template <typename T>
 struct PlaneVals {
     T lr_dx, lr_dy, lr_lap, lr_up;       // lnrho: d_x, d_y, d_xx + d_yy, |ux| up_x + |uy| up_y
     T u_dx[3], u_dy[3], u_lap[3], u_up[3];
+    T lr_upx, lr_upy, u_upx[3], u_upy[3];  // upwind terms per direction (adjoint body only)
 };
 constexpr int NDL = 10;  // delayed values kept in LDS
 
@@ -45,9 +47,12 @@ struct Lds {
     __device__ __forceinline__ T at(int dx, int dy) const { return p[dy * LX + dx]; }
 };
 
-template <typename T, bool UPWD, bool ENTROPY, int KB, int TX, int TY>
-__global__ void __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY == 256 ? 2 : 1))
-    solve_tiled_kernel(SolveArgs<T> a, int zchunk, int ntx, int nty, int nchunks)
+// The plane loop shared by the forward solve (ADJ = false: right-hand side and RK3 update of the
+// output cell) and the adjoint's local stage (ADJ = true: derivative adjoints of the output cell
+// from the same derivative quantities, with mu = lambda_w + beta lambda_f; see adjoint.hip).
+template <typename T, bool UPWD, bool ENTROPY, int KB, int TX, int TY, bool ADJ>
+__device__ __forceinline__ void delay_line_body(const SolveArgs<T>& a, const AdjLocal<T>& x,
+                                                int zchunk, int ntx, int nty, int nchunks)
 {
     constexpr int LX = TX + 2 * R;
     constexpr int LY = TY + 2 * R;
@@ -138,6 +143,7 @@ __global__ void __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY == 256 ? 2 
 
     const T dt = solve_dt(a);
     double u2max = 0.0;  // max |u_new|^2 (the courant timestep mode)
+    double pnu = 0.0, peta = 0.0;  // adjoint body: dt <mu, dR/dnu>, dt <mu, dR/deta>
     const int gx = x0 + tx, gy = y0 + ty;
     const int ctr = (ty + R) * LX + (tx + R);
     const unsigned ioff = (unsigned)((gx + R) + (int)sy * (gy + R));  // in-plane offset
@@ -157,14 +163,25 @@ __global__ void __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY == 256 ? 2 
         // stencil work (it is needed only by the RK update at the end)
         T wpre[NF];
         int cpre[NF];
+        T lfpre[NF];  // adjoint body: lambda_f and lambda_w of the output cell
         const int zo = p - R;
         const bool has_out = it >= 2 * R;
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
             wpre[f] = T(0);
             cpre[f] = 0;
+            lfpre[f] = T(0);
         }
-        if (has_out) {
+        if (ADJ && has_out) {
+            const long long poff = sz * (long long)(zo + R);
+#pragma unroll
+            for (int f = 0; f < NF; ++f) {
+                const long long fo = f * fsb + poff * (long long)sizeof(T);
+                lfpre[f] = ld_b((const T*)((const char*)x.lf + fo), iob);
+                wpre[f] = ld_b((const T*)((const char*)x.lw + fo), iob);
+            }
+        }
+        if (!ADJ && has_out) {
             const long long poff = sz * (long long)(zo + R);
             if (a.step > 0) {
 #pragma unroll
@@ -258,7 +275,7 @@ __global__ void __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY == 256 ? 2 
             d.glr[2] = dz[0];
             d.laplr = o.lr_lap + dzz[0];
             const T auz = ac_abs(q[3][3]);
-            d.upl = UPWD ? o.lr_up + auz * upz[0] : T(0);
+            d.upl = (UPWD && !ADJ) ? o.lr_up + auz * upz[0] : T(0);
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
                 d.u[i] = q[1 + i][3];
@@ -266,7 +283,7 @@ __global__ void __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY == 256 ? 2 
                 d.gu[i][1] = o.u_dy[i];
                 d.gu[i][2] = dz[1 + i];
                 d.lapu[i] = o.u_lap[i] + dzz[1 + i];
-                d.upu[i] = UPWD ? o.u_up[i] + auz * upz[1 + i] : T(0);
+                d.upu[i] = (UPWD && !ADJ) ? o.u_up[i] + auz * upz[1 + i] : T(0);
                 d.lapa[i] = dv[4 + i] + dzz[4 + i];
             }
             d.gdu[0] = ch[0][0];
@@ -290,6 +307,45 @@ __global__ void __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY == 256 ? 2 
             d.gs[2] = dz[7];
             d.laps = dv[9] + dzz[7];
 
+            if constexpr (ADJ) {
+                UpwindDirs<T> up;
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    up.ul[j] = T(0);
+#pragma unroll
+                    for (int i = 0; i < 3; ++i)
+                        up.uu[i][j] = T(0);
+                }
+                if (UPWD) {
+                    up.ul[0] = o.lr_upx;
+                    up.ul[1] = o.lr_upy;
+                    up.ul[2] = upz[0];
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) {
+                        up.uu[i][0] = o.u_upx[i];
+                        up.uu[i][1] = o.u_upy[i];
+                        up.uu[i][2] = upz[1 + i];
+                    }
+                    combine_upwind(d, up);
+                }
+                T mu[NF];
+#pragma unroll
+                for (int f = 0; f < NF; ++f)
+                    mu[f] = wpre[f] + a.beta * lfpre[f];
+                if (!ENTROPY)
+                    mu[7] = T(0);
+                CellDerivs<T> ad;
+                UpwindDirs<T> aup;
+                T anu, aeta;
+                mhd_rhs_vjp<T, UPWD, ENTROPY>(d, up, mu, a.P, ad, aup, anu, aeta);
+                pnu += (double)dt * (double)anu;
+                peta += (double)dt * (double)aeta;
+                const long long poffb = sz * (long long)(z + R) * (long long)sizeof(T);
+                put_derivative_adjoints(ad, aup, [&](int k, T v) {
+                    st_b((T*)((char*)x.g + (k * fsb + poffb)), iob, v);
+                });
+            }
+            else {
             T rhs[NF];
             mhd_rhs<T, UPWD, ENTROPY>(d, a.P, rhs);
             const long long poff = sz * (long long)(z + R);
@@ -313,6 +369,7 @@ __global__ void __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY == 256 ? 2 
                                   (double)fnew[3] * (double)fnew[3];
                 u2max = fmax(u2max, u2);
             }
+            }
         }
 
         // ---- (3) remaining in-plane derivatives of plane p
@@ -332,11 +389,12 @@ __global__ void __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY == 256 ? 2 
             }
             const T dxx = C.d2[0][0] * c0 + C.d2[0][1] * Sx[1] + C.d2[0][2] * Sx[2] + C.d2[0][3] * Sx[3];
             const T dyy = C.d2[1][0] * c0 + C.d2[1][1] * Sy[1] + C.d2[1][2] * Sy[2] + C.d2[1][3] * Sy[3];
-            T upxy = T(0);
+            T upxy = T(0), upx = T(0), upy = T(0);
             if (UPWD && f < 4) {
-                const T upx = C.up[0][0] * c0 + C.up[0][1] * Sx[1] + C.up[0][2] * Sx[2] + C.up[0][3] * Sx[3];
-                const T upy = C.up[1][0] * c0 + C.up[1][1] * Sy[1] + C.up[1][2] * Sy[2] + C.up[1][3] * Sy[3];
-                upxy = aux * upx + auy * upy;
+                upx = C.up[0][0] * c0 + C.up[0][1] * Sx[1] + C.up[0][2] * Sx[2] + C.up[0][3] * Sx[3];
+                upy = C.up[1][0] * c0 + C.up[1][1] * Sy[1] + C.up[1][2] * Sy[2] + C.up[1][3] * Sy[3];
+                if (!ADJ)
+                    upxy = aux * upx + auy * upy;
             }
             const bool need_dx = (f == 0 || f == 2 || f == 5 || f == 7);
             const bool need_dy = (f == 0 || f == 1 || f == 4 || f == 7);
@@ -350,12 +408,20 @@ __global__ void __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY == 256 ? 2 
                 nv.lr_dy = dy;
                 nv.lr_lap = dxx + dyy;
                 nv.lr_up = upxy;
+                if (ADJ) {
+                    nv.lr_upx = upx;
+                    nv.lr_upy = upy;
+                }
             }
             else if (f < 4) {
                 if (f == 1) nv.u_dy[0] = dy;
                 if (f == 2) nv.u_dx[1] = dx;
                 nv.u_lap[f - 1] = dxx + dyy;
                 nv.u_up[f - 1] = upxy;
+                if (ADJ) {
+                    nv.u_upx[f - 1] = upx;
+                    nv.u_upy[f - 1] = upy;
+                }
             }
             else if (f < 7) {
                 if (f == 4) dl[dslot][0][tid] = dy;
@@ -411,12 +477,51 @@ __global__ void __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY == 256 ? 2 
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
     }
-    if (a.step == 2) {
+    if constexpr (ADJ) {
+        // block sums of the parameter sensitivities (FP64, deterministic)
+        __shared__ double psh[2][NW];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            pnu += __shfl_xor(pnu, off, 64);
+            peta += __shfl_xor(peta, off, 64);
+        }
+        if (lane == 0) {
+            psh[0][wave] = pnu;
+            psh[1][wave] = peta;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double s0 = 0.0, s1 = 0.0;
+            for (int w = 0; w < NW; ++w) {
+                s0 += psh[0][w];
+                s1 += psh[1][w];
+            }
+            x.partials[2 * (long long)blockIdx.x] = s0;
+            x.partials[2 * (long long)blockIdx.x + 1] = s1;
+        }
+    }
+    else if (a.step == 2) {
         if (a.umax2_next != nullptr)
             wave_max_to_global(u2max, a.umax2_next);
         if (a.time_acc != nullptr && blockIdx.x == 0 && tid == 0)
             *a.time_acc += (double)dt;
     }
+}
+
+template <typename T, bool UPWD, bool ENTROPY, int KB, int TX, int TY>
+__global__ void __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY == 256 ? 2 : 1))
+    solve_tiled_kernel(SolveArgs<T> a, int zchunk, int ntx, int nty, int nchunks)
+{
+    delay_line_body<T, UPWD, ENTROPY, KB, TX, TY, false>(a, AdjLocal<T>{}, zchunk, ntx, nty, nchunks);
+}
+
+// The adjoint's local stage on 64 x 4 tiles, one wave per SIMD (the VJP needs the registers).
+template <typename T, bool UPWD, bool ENTROPY>
+__global__ void __launch_bounds__(256, 1)
+    adjoint_local_tiled_kernel(SolveArgs<T> a, AdjLocal<T> x, int zchunk, int ntx, int nty,
+                               int nchunks)
+{
+    delay_line_body<T, UPWD, ENTROPY, 0, 64, 4, true>(a, x, zchunk, ntx, nty, nchunks);
 }
 
 // Launch shapes (TX, TY, ZC): 64 x 4 and 64 x 2 run the delay-line kernel above; 32 x 8,
@@ -505,5 +610,30 @@ bool launch_solve_tiled(const SolveArgs<T>& a, bool upwd, bool entropy, int kb, 
 
 template bool launch_solve_tiled<AcRealT>(const SolveArgs<AcRealT>&, bool, bool, int, TiledShape,
                                           hipStream_t);
+
+template <typename T>
+bool launch_adjoint_local_tiled(const SolveArgs<T>& a, const AdjLocal<T>& x, bool upwd,
+                                bool entropy, hipStream_t s, long long* nblocks)
+{
+    if (a.nx % 64 != 0 || a.ny % 4 != 0 || !uniform_slabs(a, 0))
+        return false;
+    int zc = a.nz;
+    for (int cand : {32, 16, 8})
+        if (a.nz % cand == 0 && a.nz > cand) {
+            zc = cand;
+            break;
+        }
+    const int ntx = a.nx / 64, nty = a.ny / 4, nch = a.nz / zc;
+    const dim3 tpb(64, 4, 1), bpg((unsigned)(ntx * nty * nch), 1, 1);
+    *nblocks = (long long)ntx * nty * nch;
+    if (upwd && entropy) hipLaunchKernelGGL((adjoint_local_tiled_kernel<T, true, true>), bpg, tpb, 0, s, a, x, zc, ntx, nty, nch);
+    else if (upwd) hipLaunchKernelGGL((adjoint_local_tiled_kernel<T, true, false>), bpg, tpb, 0, s, a, x, zc, ntx, nty, nch);
+    else if (entropy) hipLaunchKernelGGL((adjoint_local_tiled_kernel<T, false, true>), bpg, tpb, 0, s, a, x, zc, ntx, nty, nch);
+    else hipLaunchKernelGGL((adjoint_local_tiled_kernel<T, false, false>), bpg, tpb, 0, s, a, x, zc, ntx, nty, nch);
+    return true;
+}
+
+template bool launch_adjoint_local_tiled<AcRealT>(const SolveArgs<AcRealT>&, const AdjLocal<AcRealT>&,
+                                                  bool, bool, hipStream_t, long long*);
 
 }  // namespace acamd

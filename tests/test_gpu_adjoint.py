@@ -30,7 +30,7 @@ def physics_info(lib, n, **over):
     p = dict(AC_len=(2 * np.pi,) * 3, AC_cs_sound=1.0, AC_gamma=5.0 / 3.0, AC_nu_visc=5e-3,
              AC_eta=5e-3, AC_chi=5e-3, AC_K_heatcond=5e-3, AC_lupwd=True, AC_lentropy=True)
     p.update(over)
-    return lib.config((n, n, n), **p)
+    return lib.config(n if isinstance(n, tuple) else (n, n, n), **p)
 
 
 def initial(lib, info, seed):
@@ -60,9 +60,9 @@ def lib(request, require_gpu):
 
 @pytest.mark.parametrize("step", [0, 1, 2])
 @pytest.mark.parametrize("upwd,entropy", [(True, True), (False, False), (True, False)])
-def test_substep_dot_product(lib, step, upwd, entropy):
-    say(f"case dot-product substep {step} {lib.precision} upwd={upwd} entropy={entropy}")
-    n = 16
+@pytest.mark.parametrize("n", [16, (64, 8, 16)])  # one-cell-per-thread and tiled local stage
+def test_substep_dot_product(lib, step, upwd, entropy, n):
+    say(f"case dot-product substep {step} {lib.precision} upwd={upwd} entropy={entropy} n={n}")
     info = physics_info(lib, n, AC_lupwd=upwd, AC_lentropy=entropy)
     f = initial(lib, info, 3 + step)
     # the identity is linear algebra and holds for any dt; a large one makes the Jacobian part
