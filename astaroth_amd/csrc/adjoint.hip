@@ -430,6 +430,304 @@ __global__ void __launch_bounds__(256) adjoint_transpose_z_kernel(AdjArgs<T> A, 
     }
 }
 
+// ---------------------------------------------------------------- adjoint (C), LDS-staged form
+// The same per-output-field transposes on 64 x 4 column blocks: the derivative-adjoint fields
+// output field Q reads with in-plane (x / y) neighbours are DMA'd plane by plane into LDS (with
+// their radius-3 halo, double-buffered, as the forward solve stages its planes), fields read
+// only at the column come straight from memory, and the z parts use the same push chain.
+
+// Fields of output Q that are read with in-plane neighbours ("staged"), in slot order.
+__host__ __device__ constexpr int staged_count(int Q, bool upwd)
+{
+    if (Q == 0)
+        return upwd ? 5 : 3;
+    if (Q <= 3)
+        return (upwd ? 5 : 3) + (Q == 3 ? 2 : 3);
+    if (Q <= 6)
+        return 3 + (Q == 6 ? 2 : 3);
+    return 3;
+}
+__host__ __device__ constexpr int staged_field(int Q, bool upwd, int slot)
+{
+    // lnrho / u: D1x, D1y, lap[, up x, up y]; A: D1x, D1y, lap; ss: D1x, D1y, lap;
+    // then the grad(div) fields G0, G1[, G2] of the vector (G2 only for the x and y components)
+    if (Q <= 3) {
+        const int b = Q == 0 ? G_LR : G_U + 8 * (Q - 1);
+        const int own[5] = {b + 1, b + 2, b + 4, b + 5, b + 6};
+        const int no = upwd ? 5 : 3;
+        if (slot < no)
+            return own[slot];
+        return G_GDU + (slot - no);
+    }
+    if (Q <= 6) {
+        const int b = G_A + 4 * (Q - 4);
+        const int own[3] = {b + 0, b + 1, b + 3};
+        if (slot < 3)
+            return own[slot];
+        return G_GDA + (slot - 3);
+    }
+    const int own[3] = {G_S + 1, G_S + 2, G_S + 4};
+    return own[slot];
+}
+
+// Fields of output Q read only at the column itself, in slot order.
+__host__ __device__ constexpr int column_count(int Q, bool upwd)
+{
+    if (Q <= 3)
+        return (upwd ? 3 : 2) + (Q == 3 ? 1 : 0);
+    if (Q <= 6)
+        return 1 + (Q == 6 ? 1 : 0);
+    return 2;
+}
+__host__ __device__ constexpr int column_field(int Q, bool upwd, int slot)
+{
+    // lnrho / u: val, D1z[, up z]; A: D1z; ss: val, D1z; then G2 of grad(div) for the z components
+    if (Q <= 3) {
+        const int b = Q == 0 ? G_LR : G_U + 8 * (Q - 1);
+        const int own[3] = {b + 0, b + 3, b + 7};
+        const int no = upwd ? 3 : 2;
+        return slot < no ? own[slot] : G_GDU + 2;
+    }
+    if (Q <= 6)
+        return slot == 0 ? G_A + 4 * (Q - 4) + 2 : G_GDA + 2;
+    return slot == 0 ? G_S : G_S + 3;
+}
+
+template <typename T, bool UPWD, int Q>
+__global__ void __launch_bounds__(256) adjoint_transpose_lds_kernel(AdjArgs<T> A, int zchunk,
+                                                                    int ntx, int nty)
+{
+    constexpr int TX = 64, TY = 4;
+    constexpr int LX = TX + 2 * R, LY = TY + 2 * R;
+    constexpr int LP = LX * LY;
+    constexpr int NT = TX * TY, NW = NT / 64;
+    constexpr int WPE = (int)sizeof(T) / 4;
+    constexpr int LPW = LP * WPE;
+    constexpr int NCHUNK = (LPW + 63) / 64;
+    constexpr int NR = (NCHUNK + NW - 1) / NW;
+    constexpr int LPS = NR * NW * 64 / WPE;  // padded field-plane stride in LDS (elements)
+    constexpr int NS = staged_count(Q, UPWD);
+    constexpr int NC = column_count(Q, UPWD);
+    __shared__ T lds[2][NS * LPS];
+
+    const SolveArgs<T>& a = A.s;
+    // XCD-aware numbering: consecutive hardware block ids go to different XCDs, so each XCD is
+    // handed a contiguous range of tiles (whose y neighbours then share its L2)
+    const int nblocks = (int)gridDim.x;
+    int bid = (int)blockIdx.x;
+    if ((nblocks & 7) == 0)
+        bid = (bid & 7) * (nblocks >> 3) + (bid >> 3);
+    const int tile_x = bid % ntx;
+    const int rest = bid / ntx;
+    const int tile_y = rest % nty;
+    const int chunk = rest / nty;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int tid = tx + TX * ty;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int x0 = tile_x * TX, y0 = tile_y * TY, z0 = chunk * zchunk;
+    const int sy = a.mx;
+    const long long sz = a.sz;
+    const Coeffs<T>& C = a.C;
+
+    // DMA: every lane moves one dword per chunk; padding lanes re-read dword 0 of the plane.
+    // The x / y halo of a tile is read from the periodic images inside the interior, so the
+    // derivative adjoints need no halo fill.
+    unsigned g_off[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        int d = (wave + r * NW) * 64 + lane;
+        if (d >= LPW)
+            d = 0;
+        const int e = d / WPE, wd = d - e * WPE;
+        const int ly = e / LX, lx = e - ly * LX;
+        int gx = x0 + lx - R, gy = y0 + ly - R;
+        gx += gx < 0 ? a.nx : (gx >= a.nx ? -a.nx : 0);
+        gy += gy < 0 ? a.ny : (gy >= a.ny ? -a.ny : 0);
+        g_off[r] = (unsigned)(((gx + R) + sy * (gy + R)) * (int)sizeof(T) + 4 * wd);
+    }
+    // byte offset of interior plane p (z wrapped periodically) within a field
+    auto plane_b = [&](int p) {
+        p += p < 0 ? a.nz : (p >= a.nz ? -a.nz : 0);
+        return sz * (long long)(p + R) * (long long)sizeof(T);
+    };
+    auto dma_plane = [&](int p, int b, long long nb) {
+        const char* pb = (const char*)A.g + plane_b(p);
+#pragma unroll
+        for (int sl = 0; sl < NS; ++sl) {
+            const char* src = pb + staged_field(Q, UPWD, sl) * nb;
+            char* dst = (char*)&lds[b][sl * LPS];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                const int j = wave + r * NW;
+                __builtin_amdgcn_global_load_lds((const void*)(src + opaque(g_off[r])),
+                                                 (lds_ptr_t)(dst + 256 * j), 4, 0, 0);
+            }
+        }
+    };
+
+    const int ctr = (ty + R) * LX + (tx + R);
+    const unsigned colb0 = (unsigned)(((x0 + tx + R) + sy * (y0 + ty + R)) * (int)sizeof(T));
+    T acc[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k)
+        acc[k] = T(0);
+    // column values of the plane being processed, and of the next one (loaded a plane ahead)
+    T col[NC], coln[NC];
+    // adjoint state of the plane completed in this step (lfc, lwc) and in the next one (loaded a
+    // plane ahead)
+    T lfc = T(0), lwc = T(0), lfn = T(0), lwn = T(0);
+    auto load_cols = [&](int p, long long nb, unsigned colb, T* dst) {
+        const char* pb = (const char*)A.g + plane_b(p);
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+            dst[c] = *(const T*)(pb + column_field(Q, UPWD, c) * nb + colb);
+    };
+    // the output plane of step it is p - 3 = z0 - 2R + it (array plane z0 - R + it)
+    auto out_off = [&](int it, long long nb, unsigned colb) {
+        return Q * nb + sz * (long long)(z0 - R + it) * (long long)sizeof(T) + colb;
+    };
+
+    {
+        const long long nb = A.count * (long long)sizeof(T);
+        dma_plane(z0 - R, 0, nb);
+        load_cols(z0 - R, nb, colb0, col);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    const int np = zchunk + 2 * R;
+    for (int it = 0; it < np; ++it) {
+        const int p = z0 - R + it;
+        const int cur = it & 1;
+        const long long nb = opaque_s64(A.count * (long long)sizeof(T));
+        const unsigned colb = opaque(colb0);
+        if (it + 1 < np) {
+            dma_plane(p + 1, cur ^ 1, nb);
+            load_cols(p + 1, nb, colb, coln);
+            if (it + 1 >= 2 * R) {
+                const long long ob = out_off(it + 1, nb, colb);
+                lfn = *(const T*)((const char*)A.lf + ob);
+                lwn = *(const T*)((const char*)A.lw + ob);
+            }
+        }
+        auto lds_at = [&](int sl, int dx, int dy) -> T {
+            return lds[cur][sl * LPS + ctr + dy * LX + dx];
+        };
+        auto d1x = [&](int sl) {
+            return C.d1[0][1] * (lds_at(sl, 1, 0) - lds_at(sl, -1, 0)) +
+                   C.d1[0][2] * (lds_at(sl, 2, 0) - lds_at(sl, -2, 0)) +
+                   C.d1[0][3] * (lds_at(sl, 3, 0) - lds_at(sl, -3, 0));
+        };
+        auto d1y = [&](int sl) {
+            return C.d1[1][1] * (lds_at(sl, 0, 1) - lds_at(sl, 0, -1)) +
+                   C.d1[1][2] * (lds_at(sl, 0, 2) - lds_at(sl, 0, -2)) +
+                   C.d1[1][3] * (lds_at(sl, 0, 3) - lds_at(sl, 0, -3));
+        };
+        auto sym = [&](int sl, int ax, const T* w) {
+            const int ex = ax == 0, ey = ax == 1;
+            return w[0] * lds_at(sl, 0, 0) + w[1] * (lds_at(sl, ex, ey) + lds_at(sl, -ex, -ey)) +
+                   w[2] * (lds_at(sl, 2 * ex, 2 * ey) + lds_at(sl, -2 * ex, -2 * ey)) +
+                   w[3] * (lds_at(sl, 3 * ex, 3 * ey) + lds_at(sl, -3 * ex, -3 * ey));
+        };
+        auto mixed_xy = [&](int sl) {
+            T r = T(0);
+#pragma unroll
+            for (int o = 1; o <= 3; ++o)
+                r += C.mx[0][o] * ((lds_at(sl, o, o) + lds_at(sl, -o, -o)) -
+                                   (lds_at(sl, o, -o) + lds_at(sl, -o, o)));
+            return r;
+        };
+        auto push_anti = [&](T v) {
+#pragma unroll
+            for (int o = 1; o <= 3; ++o) {
+                acc[3 - o] -= C.d1[2][o] * v;
+                acc[3 + o] += C.d1[2][o] * v;
+            }
+        };
+        auto push_sym = [&](T v, const T* w) {
+            acc[3] += w[0] * v;
+#pragma unroll
+            for (int o = 1; o <= 3; ++o) {
+                acc[3 - o] += w[o] * v;
+                acc[3 + o] += w[o] * v;
+            }
+        };
+        auto push_mixed = [&](int sl, int ax) {
+            const int ex = ax == 0, ey = ax == 1;
+            const T* w = C.mx[ax == 0 ? 1 : 2];
+#pragma unroll
+            for (int o = 1; o <= 3; ++o) {
+                const T D = w[o] * (lds_at(sl, o * ex, o * ey) - lds_at(sl, -o * ex, -o * ey));
+                acc[3 - o] += D;
+                acc[3 + o] -= D;
+            }
+        };
+
+        // own stencils: staged slots 0: D1x, 1: D1y, 2: lap[, 3: up x, 4: up y]; column slots
+        // in column_field order
+        if (Q < 4) {
+            T r = col[0] - (d1x(0) + d1y(1));
+            r += sym(2, 0, C.d2[0]) + sym(2, 1, C.d2[1]);
+            if (UPWD)
+                r += sym(3, 0, C.up[0]) + sym(4, 1, C.up[1]);
+            acc[3] += r;
+            push_anti(col[1]);
+            push_sym(lds_at(2, 0, 0), C.d2[2]);
+            if (UPWD)
+                push_sym(col[2], C.up[2]);
+        }
+        else if (Q < 7) {
+            acc[3] += sym(2, 0, C.d2[0]) + sym(2, 1, C.d2[1]) - (d1x(0) + d1y(1));
+            push_anti(col[0]);
+            push_sym(lds_at(2, 0, 0), C.d2[2]);
+        }
+        else {
+            T r = col[0] - (d1x(0) + d1y(1));
+            r += sym(2, 0, C.d2[0]) + sym(2, 1, C.d2[1]);
+            acc[3] += r;
+            push_anti(col[1]);
+            push_sym(lds_at(2, 0, 0), C.d2[2]);
+        }
+        // grad(div v) parts (staged slots from the own-stencil count: G0, G1[, G2])
+        if (Q >= 1 && Q <= 6) {
+            constexpr int no = Q <= 3 ? (UPWD ? 5 : 3) : 3;
+            const int comp = (Q - 1) % 3;
+            if (comp == 0) {
+                acc[3] += sym(no + 0, 0, C.d2[0]) + mixed_xy(no + 1);
+                push_mixed(no + 2, 0);  // d_xz of G2
+            }
+            else if (comp == 1) {
+                acc[3] += mixed_xy(no + 0) + sym(no + 1, 1, C.d2[1]);
+                push_mixed(no + 2, 1);  // d_yz of G2
+            }
+            else {
+                push_mixed(no + 0, 0);  // d_xz of G0
+                push_mixed(no + 1, 1);  // d_yz of G1
+                push_sym(col[NC - 1], C.d2[2]);
+            }
+        }
+
+        if (it >= 2 * R) {
+            const long long ob = out_off(it, nb, colb);
+            const T lfo = lfc;
+            const T mu = lwc + a.beta * lfo;
+            *(T*)((char*)A.lf + ob) = lfo + a.dt * acc[0];
+            *(T*)((char*)A.lw + ob) = a.alpha * mu;
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            acc[k] = acc[k + 1];
+        acc[6] = T(0);
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+            col[c] = coln[c];
+        lfc = lfn;
+        lwc = lwn;
+        // plane p + 1 has landed and every wave is done with this one
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+    }
+}
+
 // Without the entropy equation ss is not evolved (f'_7 = f_7): its adjoint passes through and
 // lambda_w of ss is zero.
 template <typename T>
@@ -492,25 +790,33 @@ void launch_adjoint(const AdjArgs<T>& A, bool upwd, bool entropy, double* param_
                        (const double*)A.partials, nb, ranges);
     hipLaunchKernelGGL(adjoint_param_stage2, dim3(1), dim3(256), 0, s, (const double*)ranges,
                        param_grad);
-    // (B) periodic fill of the derivative adjoints, eight fields per launch
-    for (int k0 = 0; k0 < NG; k0 += NF) {
-        T* ptrs[NF];
-        const int cnt = NG - k0 < NF ? NG - k0 : NF;
-        for (int q = 0; q < cnt; ++q)
-            ptrs[q] = A.g + (long long)(k0 + q) * A.count;
-        launch_periodic_fill<T>(ptrs, cnt, A.s.nx, A.s.ny, A.s.nz, s);
-    }
-    // (C) transposed stencils, z-marching over chunks of planes
+    // (C) transposed stencils, z-marching over chunks of planes: the LDS-staged form where the
+    // grid fits its 64 x 4 column blocks (it reads the periodic images itself), else the plain
+    // form after (B) a periodic fill of the derivative adjoints
     int zc = A.s.nz;
     for (int cand : {32, 16, 8})
         if (A.s.nz % cand == 0 && A.s.nz > cand) {
             zc = cand;
             break;
         }
+    const bool staged = A.s.nx % 64 == 0 && A.s.ny % 4 == 0 && A.s.nz >= R;
+    for (int k0 = 0; !staged && k0 < NG; k0 += NF) {
+        T* ptrs[NF];
+        const int cnt = NG - k0 < NF ? NG - k0 : NF;
+        for (int q = 0; q < cnt; ++q)
+            ptrs[q] = A.g + (long long)(k0 + q) * A.count;
+        launch_periodic_fill<T>(ptrs, cnt, A.s.nx, A.s.ny, A.s.nz, s);
+    }
     const dim3 bpz(bpg.x, bpg.y, (unsigned)(A.s.nz / zc));
+    const int ntx = A.s.nx / 64, nty = A.s.ny / 4;
+    const dim3 bps((unsigned)(ntx * nty * (A.s.nz / zc)), 1, 1);
 #define AC_ADJ_Q(Q)                                                                                \
     do {                                                                                           \
-        if (upwd) hipLaunchKernelGGL((adjoint_transpose_z_kernel<T, true, Q>), bpz, tpb, 0, s, A, zc); \
+        if (staged) {                                                                              \
+            if (upwd) hipLaunchKernelGGL((adjoint_transpose_lds_kernel<T, true, Q>), bps, tpb, 0, s, A, zc, ntx, nty); \
+            else hipLaunchKernelGGL((adjoint_transpose_lds_kernel<T, false, Q>), bps, tpb, 0, s, A, zc, ntx, nty);     \
+        }                                                                                          \
+        else if (upwd) hipLaunchKernelGGL((adjoint_transpose_z_kernel<T, true, Q>), bpz, tpb, 0, s, A, zc); \
         else hipLaunchKernelGGL((adjoint_transpose_z_kernel<T, false, Q>), bpz, tpb, 0, s, A, zc);    \
     } while (0)
     AC_ADJ_Q(0);
