@@ -72,6 +72,7 @@ struct device_s {
     int cfl_cur;
     double cfl_courant;
     int cfl_mode;                 // 0 courant, 1 reference (2 when AC_additive_timestep)
+    bool periodic_fused;          // substeps read the periodic images (no halo fill before them)
     double* host_pinned;          // staging for small D2H reads
     // launch shape
     TiledShape shape;
@@ -197,14 +198,22 @@ static void fill_solve_args(Device d, SolveArgs<AcReal>& a, int step, AcReal dt)
     a.cfl_dsmin = (double)info.real_params[AC_dsmin];
     a.cfl_cs = (double)info.real_params[AC_cs_sound];
     set_cfl_args(d, a);
+    a.wrap = d->periodic_fused ? 1 : 0;
 }
 
-static AcResult launch_solve(Device d, const SolveArgs<AcReal>& a, hipStream_t s)
+static AcResult launch_solve(Device d, const SolveArgs<AcReal>& a_in, hipStream_t s)
 {
     const AcMeshInfo& info = d->local_config;
     const bool upwd = info.bool_params[AC_lupwd] != 0;
     const bool entropy = info.bool_params[AC_lentropy] != 0;
     const int kb = d->kahan_bits;
+    SolveArgs<AcReal> a = a_in;
+    if (a.wrap && !(d->shape.tx > 0 && tiled_reads_periodic<AcReal>(a, kb, d->shape))) {
+        // the kernel that runs reads the halos: fill them first
+        const int3 n = info.int3_params[AC_nlocal];
+        launch_periodic_fill<AcReal>(d->in, NF, n.x, n.y, n.z, s);
+        a.wrap = 0;
+    }
     const bool timed = d->timing && d->events_created && d->timing_used < kTimingSlots;
     if (timed)
         AC_HIP_CHECK(hipEventRecord(d->ev_start[d->timing_used], s));
@@ -943,10 +952,23 @@ AcResult acDeviceIntegrateStepsCFL(const Device d, const Stream stream, const in
     const int3 n = d->local_config.int3_params[AC_nlocal];
     if (time_out)
         launch_zero_bytes(d->time_acc, sizeof(double), s);
+    // The tiled solve kernels read the periodic images of the halo cells themselves, so no fill
+    // precedes their substeps (launch_solve fills first where another kernel runs). The reference
+    // timestep modes reduce over the state entering substep 2 with derivatives that read the
+    // halos: that substep keeps its fill. One fill up front also gives the launch autotuner
+    // (first substep on a new grid) a state with consistent halos.
+    launch_periodic_fill<AcReal>(d->in, NF, n.x, n.y, n.z, s);
+    if (ensure_shape(d, s) != AC_SUCCESS)
+        return AC_FAILURE;
     for (int step = 0; step < nsteps; ++step) {
         for (int sub = 0; sub < 3; ++sub) {
-            launch_periodic_fill<AcReal>(d->in, NF, n.x, n.y, n.z, s);
-            if (acDeviceIntegrateSubstepDeviceDt(d, stream, sub, courant) != AC_SUCCESS)
+            const bool fused = d->cfl_mode == 0 || sub != 2;
+            if (!fused)
+                launch_periodic_fill<AcReal>(d->in, NF, n.x, n.y, n.z, s);
+            d->periodic_fused = fused;
+            const AcResult r = acDeviceIntegrateSubstepDeviceDt(d, stream, sub, courant);
+            d->periodic_fused = false;
+            if (r != AC_SUCCESS)
                 return AC_FAILURE;
             acDeviceSwapBuffers(d);
         }

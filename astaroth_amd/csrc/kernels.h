@@ -83,6 +83,10 @@ __device__ __forceinline__ long long opaque_s64(long long v)
 int tiled_num_shapes();
 TiledShape tiled_shape(int i);
 bool tiled_shape_fits(TiledShape s, int nx, int ny, int nz);
+// Whether launch_solve_tiled with this shape runs a delay-line kernel that honours a.wrap (reads
+// the periodic images instead of the halos).
+template <typename T>
+bool tiled_reads_periodic(const SolveArgs<T>& a, int kb, TiledShape sh);
 
 // ---- seeded Fourier-mode initial condition (ic.hip) ----
 template <typename T>

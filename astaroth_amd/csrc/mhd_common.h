@@ -64,6 +64,9 @@ struct SolveArgs {
     double cfl_cdt, cfl_dsmin, cfl_cs;
     int cfl_mode;                    // 0 courant, 1 reference, 2 reference additive
     double cfl_cs2, cfl_cdtv, cfl_visc;  // cs^2, cdtv, max(nu, eta[, gamma chi])
+    // 1: the tiled (delay-line) kernels read the x / y / z halo of `in` from the periodic images
+    // inside the interior, so the halos need not be filled; 0: they read the halos as stored
+    int wrap;
 };
 
 __device__ __forceinline__ float ac_exp(float x) { return expf(x); }

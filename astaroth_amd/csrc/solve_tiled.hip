@@ -50,7 +50,7 @@ struct Lds {
 // The plane loop shared by the forward solve (ADJ = false: right-hand side and RK3 update of the
 // output cell) and the adjoint's local stage (ADJ = true: derivative adjoints of the output cell
 // from the same derivative quantities, with mu = lambda_w + beta lambda_f; see adjoint.hip).
-template <typename T, bool UPWD, bool ENTROPY, int KB, int TX, int TY, bool ADJ>
+template <typename T, bool UPWD, bool ENTROPY, int KB, int TX, int TY, bool ADJ, bool WRAP>
 __device__ __forceinline__ void delay_line_body(const SolveArgs<T>& a, const AdjLocal<T>& x,
                                                 int zchunk, int ntx, int nty, int nchunks)
 {
@@ -101,7 +101,13 @@ __device__ __forceinline__ void delay_line_body(const SolveArgs<T>& a, const Adj
             d = 0;  // padding lane: re-read the first dword (in bounds, value unused)
         const int e = d / WPE, wd = d - e * WPE;
         const int ly = e / LX, lx = e - ly * LX;
-        g_off[r] = (unsigned)(((x0 + lx) + (int)sy * (y0 + ly)) * (int)sizeof(T) + 4 * wd);
+        int hx = x0 + lx, hy = y0 + ly;  // array coordinates of the element
+        if (WRAP) {
+            // halo elements from their periodic images in the interior
+            hx += hx < R ? a.nx : (hx >= a.nx + R ? -a.nx : 0);
+            hy += hy < R ? a.ny : (hy >= a.ny + R ? -a.ny : 0);
+        }
+        g_off[r] = (unsigned)((hx + (int)sy * hy) * (int)sizeof(T) + 4 * wd);
     }
     // Field f of a role lives at slab + f * fstride; the per-field bases are recomputed from a
     // stride made opaque once per plane step (scalar work) rather than held across the loop.
@@ -111,6 +117,8 @@ __device__ __forceinline__ void delay_line_body(const SolveArgs<T>& a, const Adj
     char* const comp0 = (char*)a.comp[0];
     // Issue the DMA of interior plane p (array z index p + R) into buffer b.
     auto dma_plane = [&](int p, int b, long long fsb) {
+        if (WRAP)
+            p += p < 0 ? a.nz : (p >= a.nz ? -a.nz : 0);
         const long long base = sz * (long long)(p + R) * (long long)sizeof(T);
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
@@ -508,11 +516,14 @@ __device__ __forceinline__ void delay_line_body(const SolveArgs<T>& a, const Adj
     }
 }
 
-template <typename T, bool UPWD, bool ENTROPY, int KB, int TX, int TY>
+// WRAP: read the halo from the periodic images inside the interior (a.wrap), a separate
+// instantiation so that the plain kernels keep their own code generation.
+template <typename T, bool UPWD, bool ENTROPY, int KB, int TX, int TY, bool WRAP>
 __global__ void __launch_bounds__(TX* TY, (sizeof(T) == 4 && TX * TY == 256 ? 2 : 1))
     solve_tiled_kernel(SolveArgs<T> a, int zchunk, int ntx, int nty, int nchunks)
 {
-    delay_line_body<T, UPWD, ENTROPY, KB, TX, TY, false>(a, AdjLocal<T>{}, zchunk, ntx, nty, nchunks);
+    delay_line_body<T, UPWD, ENTROPY, KB, TX, TY, false, WRAP>(a, AdjLocal<T>{}, zchunk, ntx, nty,
+                                                               nchunks);
 }
 
 // The adjoint's local stage on 64 x 4 tiles, one wave per SIMD (the VJP needs the registers).
@@ -521,7 +532,7 @@ __global__ void __launch_bounds__(256, 1)
     adjoint_local_tiled_kernel(SolveArgs<T> a, AdjLocal<T> x, int zchunk, int ntx, int nty,
                                int nchunks)
 {
-    delay_line_body<T, UPWD, ENTROPY, 0, 64, 4, true>(a, x, zchunk, ntx, nty, nchunks);
+    delay_line_body<T, UPWD, ENTROPY, 0, 64, 4, true, false>(a, x, zchunk, ntx, nty, nchunks);
 }
 
 // Launch shapes (TX, TY, ZC): 64 x 4 and 64 x 2 run the delay-line kernel above; 32 x 8,
@@ -559,8 +570,12 @@ static void launch_shape(const SolveArgs<T>& a, int zchunk, hipStream_t s)
     const int ntx = a.nx / TX, nty = a.ny / TY, nch = a.nz / zchunk;
     const dim3 tpb(TX, TY, 1);
     const dim3 bpg((unsigned)(ntx * nty * nch), 1, 1);
-    hipLaunchKernelGGL((solve_tiled_kernel<T, UPWD, ENTROPY, KB, TX, TY>), bpg, tpb, 0, s, a,
-                       zchunk, ntx, nty, nch);
+    if (a.wrap)
+        hipLaunchKernelGGL((solve_tiled_kernel<T, UPWD, ENTROPY, KB, TX, TY, true>), bpg, tpb, 0,
+                           s, a, zchunk, ntx, nty, nch);
+    else
+        hipLaunchKernelGGL((solve_tiled_kernel<T, UPWD, ENTROPY, KB, TX, TY, false>), bpg, tpb, 0,
+                           s, a, zchunk, ntx, nty, nch);
 }
 
 template <typename T, bool UPWD, bool ENTROPY, int KB>
@@ -587,10 +602,19 @@ static bool uniform_slabs(const SolveArgs<T>& a, int kb)
 }
 
 template <typename T>
+bool tiled_reads_periodic(const SolveArgs<T>& a, int kb, TiledShape sh)
+{
+    return tiled_shape_fits(sh, a.nx, a.ny, a.nz) && uniform_slabs(a, kb) && !is_ring(sh) &&
+           a.nx >= R && a.ny >= R && a.nz >= R;
+}
+
+template <typename T>
 bool launch_solve_tiled(const SolveArgs<T>& a, bool upwd, bool entropy, int kb, TiledShape sh,
                         hipStream_t s)
 {
     if (!tiled_shape_fits(sh, a.nx, a.ny, a.nz) || !uniform_slabs(a, kb))
+        return false;
+    if (a.wrap && !tiled_reads_periodic(a, kb, sh))
         return false;
     if (is_ring(sh))
         return launch_solve_ring<T>(a, upwd, entropy, kb, sh, s);
@@ -610,6 +634,7 @@ bool launch_solve_tiled(const SolveArgs<T>& a, bool upwd, bool entropy, int kb, 
 
 template bool launch_solve_tiled<AcRealT>(const SolveArgs<AcRealT>&, bool, bool, int, TiledShape,
                                           hipStream_t);
+template bool tiled_reads_periodic<AcRealT>(const SolveArgs<AcRealT>&, int, TiledShape);
 
 template <typename T>
 bool launch_adjoint_local_tiled(const SolveArgs<T>& a, const AdjLocal<T>& x, bool upwd,

@@ -399,9 +399,12 @@ AcResult acDeviceGetTimestep(const Device device, const Stream stream, double* d
  * the kernel also reduces max|u| of the new state and sets the next dt (courant as given). */
 AcResult acDeviceIntegrateSubstepDeviceDt(const Device device, const Stream stream,
                                           const int step_number, const AcReal courant);
-/* nsteps full RK3 steps (periodic fill + fused substep + swap, three times per step) with
- * the device-resident CFL dt; no host synchronisation inside. Accumulated simulated time is
- * added to *time_out when it is not NULL (synchronises at the end). */
+/* nsteps full RK3 steps (periodic boundaries + fused substep + swap, three times per step) with
+ * the device-resident CFL dt; no host synchronisation inside. The tiled solve kernels read the
+ * periodic images of the halo cells themselves (no separate fill before their substeps; other
+ * kernels get one), so the result equals fill + substep + swap bit for bit, and the halos of the
+ * final state are not refreshed (apply acDevicePeriodicBoundconds before reading them).
+ * Accumulated simulated time is added to *time_out when it is not NULL (synchronises at the end). */
 AcResult acDeviceIntegrateStepsCFL(const Device device, const Stream stream, const int nsteps,
                                    const AcReal courant, double* time_out);
 /* One pass over the interior: per-field min/max/rms, |u| and |curl A| rms/max (FP64). */

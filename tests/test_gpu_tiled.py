@@ -133,3 +133,46 @@ def test_tiled_kahan_closer_to_fp64(kb, shape, require_gpu):
     worst_p = max(e["maxrel"] for e in ep)
     worst_c = max(e["maxrel"] for e in ec)
     assert worst_c < worst_p
+
+
+@pytest.mark.parametrize("n,shape", [((64, 16, 32), (64, 4, 16)), ((64, 8, 16), (64, 2, 8)),
+                                     ((128, 4, 8), (64, 4, 8)), ((32, 32, 32), (32, 8, 32)),
+                                     ((64, 16, 32), (-1, 0, 0))])
+@pytest.mark.parametrize("mode", ["courant", "reference"])
+def test_fused_periodic_steps(lib, n, shape, mode):
+    """acDeviceIntegrateStepsCFL runs the delay-line kernels on the periodic images of the halo
+    cells (no fill before a substep); the ring and direct kernels get a fill first. Either way the
+    result equals fill + substep + swap three times per step, bit for bit."""
+    say(f"case fused periodic steps {lib.precision} n={n} shape={shape} mode={mode}")
+    info = physics_info(lib, n, shape)
+    data = initial(lib, info, seed=11)
+    courant = lib.np_real(0.3)
+    tmode = ac.AC_TIMESTEP_COURANT if mode == "courant" else ac.AC_TIMESTEP_REFERENCE
+    outs = []
+    for fused in (True, False):
+        mesh = ac.HostMesh(lib, info, data)
+        with ac.Device(lib, info) as dev:
+            dev.load_mesh(mesh)
+            ac.check(lib.acDeviceSetTimestepMode(dev.handle, tmode), "mode")
+            dev.periodic()
+            ac.check(lib.acDeviceUpdateTimestepCFL(dev.handle, 0, courant), "cfl")
+            if fused:
+                ac.check(lib.acDeviceIntegrateStepsCFL(dev.handle, 0, 2, courant, None), "steps")
+            else:
+                for _ in range(2):
+                    for s in range(3):
+                        dev.periodic()
+                        ac.check(lib.acDeviceIntegrateSubstepDeviceDt(dev.handle, 0, s, courant),
+                                 "substep")
+                        dev.swap()
+            dev.periodic()
+            out = ac.HostMesh(lib, info)
+            dev.store_mesh(out)
+            dt = ctypes.c_double()
+            ac.check(lib.acDeviceGetTimestep(dev.handle, 0, ctypes.byref(dt)), "dt")
+            outs.append((out.data.copy(), dt.value))
+    diff = np.argwhere(outs[0][0] != outs[1][0])
+    say(f"  differing values: {len(diff)}  dt {outs[0][1]:.17g} {outs[1][1]:.17g}")
+    assert len(diff) == 0, diff[:5]
+    assert outs[0][1] == outs[1][1]
+    assert np.isfinite(outs[0][0]).all()
