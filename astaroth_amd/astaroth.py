@@ -25,14 +25,19 @@ FIELD_NAMES = ["VTXBUF_LNRHO", "VTXBUF_UUX", "VTXBUF_UUY", "VTXBUF_UUZ", "VTXBUF
                "VTXBUF_AY", "VTXBUF_AZ", "VTXBUF_ENTROPY"]
 
 INT_PARAMS = ["AC_max_steps", "AC_save_steps", "AC_start_step", "AC_kahan_bits",
-              "AC_solve_tile_x", "AC_solve_tile_y", "AC_solve_zchunk"]
+              "AC_solve_tile_x", "AC_solve_tile_y", "AC_solve_zchunk", "AC_forcing_period_steps"]
 INT3_PARAMS = ["AC_ngrid", "AC_nlocal", "AC_mlocal", "AC_mgrid", "AC_nmin", "AC_nlocal_max",
                "AC_ngrid_max", "AC_multigpu_offset"]
 REAL_PARAMS = ["AC_cdt", "AC_cdtv", "AC_cdts", "AC_nu_visc", "AC_cs_sound", "AC_zeta", "AC_eta",
                "AC_mu0", "AC_chi", "AC_cp_sound", "AC_gamma", "AC_lnT0", "AC_lnrho0",
-               "AC_K_heatcond", "AC_cs2_sound", "AC_dsmin", "AC_current_time", "AC_dt"]
+               "AC_K_heatcond", "AC_cs2_sound", "AC_dsmin", "AC_current_time", "AC_dt",
+               "AC_forcing_magnitude", "AC_relhel", "AC_kmin", "AC_kmax", "AC_forcing_phase",
+               "AC_k_forcex", "AC_k_forcey", "AC_k_forcez", "AC_ff_hel_rex", "AC_ff_hel_rey",
+               "AC_ff_hel_rez", "AC_ff_hel_imx", "AC_ff_hel_imy", "AC_ff_hel_imz", "AC_kaver",
+               "AC_switch_forcing", "AC_forcing_period_t"]
 REAL3_PARAMS = ["AC_ds", "AC_inv_ds", "AC_inv_ds_2", "AC_len"]
-BOOL_PARAMS = ["AC_lupwd", "AC_lentropy", "AC_additive_timestep", "AC_periodic_grid_all"]
+BOOL_PARAMS = ["AC_lupwd", "AC_lentropy", "AC_additive_timestep", "AC_periodic_grid_all",
+               "AC_lforcing"]
 
 for _names in (INT_PARAMS, INT3_PARAMS, REAL_PARAMS, REAL3_PARAMS, BOOL_PARAMS):
     for _i, _n in enumerate(_names):
@@ -92,7 +97,8 @@ EXPORTED = [
     "acDeviceLoadMeshFromDevicePtr", "acDeviceStoreMeshToDevicePtr", "acDeviceInitFourier",
     "acHostAppendTimeseries", "acGetVertexBufferName", "acDeviceSetTimestepMode",
     "acDeviceWriteSnapshot", "acDeviceReadSnapshot", "acHostAppendSnapshotInfo",
-    "acHostReadSnapshotInfo",
+    "acHostReadSnapshotInfo", "seed_rng", "random_uniform_real_01", "generateHelicalForcingParams",
+    "loadForcingParamsToMeshInfo", "printForcingParams", "acDeviceLoadForcingParams",
 ]
 
 
@@ -207,6 +213,14 @@ class AstarothLib:
         self.c_real = ctypes.c_float if precision == "fp32" else ctypes.c_double
         self.np_real = np.float32 if precision == "fp32" else np.float64
         self.Real3, self.AcMeshInfo, self.AcMesh = _make_types(self.c_real)
+        R3_ = self.Real3
+
+        class ForcingParams(ctypes.Structure):
+            """ref include/astaroth_forcing.h ForcingParams"""
+            _fields_ = [("magnitude", self.c_real), ("k_force", R3_), ("ff_hel_re", R3_),
+                        ("ff_hel_im", R3_), ("phase", self.c_real), ("kaver", self.c_real)]
+
+        self.ForcingParams = ForcingParams
         if self.lib.acGetRealSize() != ctypes.sizeof(self.c_real):
             raise RuntimeError("AcReal size mismatch between library and binding")
         self._declare()
@@ -291,6 +305,8 @@ class AstarothLib:
             "acDeviceReadSnapshot": [dev, i, ctypes.c_char_p, ctypes.c_char_p],
             "acHostAppendSnapshotInfo": [ctypes.c_char_p, i, i, ctypes.c_double, P(Info)],
             "acHostReadSnapshotInfo": [ctypes.c_char_p, P(i), P(i), P(ctypes.c_double)],
+            "loadForcingParamsToMeshInfo": [self.ForcingParams, P(Info)],
+            "acDeviceLoadForcingParams": [dev, i, self.ForcingParams],
         }
         sig["acHostAppendTimeseries"] = [ctypes.c_char_p, i, ctypes.c_double, ctypes.c_double,
                                          ctypes.POINTER(AcDiagnostics), ctypes.POINTER(ctypes.c_int)]
@@ -303,6 +319,18 @@ class AstarothLib:
             setattr(self, name, fn)
         L.acGetVertexBufferName.argtypes = [i]
         L.acGetVertexBufferName.restype = ctypes.c_char_p
+        if hasattr(L, "generateHelicalForcingParams"):
+            L.seed_rng.argtypes = [ctypes.c_uint32]
+            L.seed_rng.restype = None
+            L.random_uniform_real_01.argtypes = []
+            L.random_uniform_real_01.restype = R
+            L.generateHelicalForcingParams.argtypes = [R, R, R, R]
+            L.generateHelicalForcingParams.restype = self.ForcingParams
+            L.printForcingParams.argtypes = [self.ForcingParams]
+            L.printForcingParams.restype = None
+            for name in ("seed_rng", "random_uniform_real_01", "generateHelicalForcingParams",
+                         "printForcingParams"):
+                setattr(self, name, getattr(L, name))
         for name in ("acDeviceGetConfig", "acDeviceGetLocalConfig"):
             fn = getattr(L, name)
             fn.argtypes = [dev]

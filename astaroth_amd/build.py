@@ -16,7 +16,7 @@ LIBDIR = os.path.join(HERE, "lib")
 OBJDIR = os.path.join(HERE, "_build")
 
 SOURCES = ["device.hip", "solve_direct.hip", "solve_tiled.hip", "solve_ring.hip",
-           "aux_kernels.hip", "ic.hip", "host.cc", "adjoint.hip"]
+           "aux_kernels.hip", "ic.hip", "host.cc", "adjoint.hip", "forcing.cc"]
 ARCH = os.environ.get("ASTAROTH_OFFLOAD_ARCH", "gfx950")
 
 HIP_FLAGS = [

@@ -472,6 +472,59 @@ void launch_cfl_state(const T* const* fields, const Coeffs<T>& C, int nx, int ny
 template void launch_cfl_state<AcRealT>(const AcRealT* const*, const Coeffs<AcRealT>&, int, int, int,
                                          int, double, unsigned long long*, hipStream_t);
 
+// ---------------------------------------------------------------- helical forcing
+// ref mhdsolver.ac:286-331 (helical_forcing, forcing) and :614-631 (added to u after the final
+// update of substep 2). The geometry and the mode are evaluated in FP64, the force is rounded to
+// AcReal and added to the stored u in AcReal, as the reference adds forcing_step to rk3_final.
+template <typename T>
+__global__ void __launch_bounds__(256) forcing_kernel(ForcingArgs<T> a)
+{
+    const int i = threadIdx.x + blockIdx.x * blockDim.x;
+    const int j = threadIdx.y + blockIdx.y * blockDim.y;
+    const int k = blockIdx.z;
+    const bool inside = i < a.nx && j < a.ny && k < a.nz;
+    const double dt = (double)solve_dt(a.s);
+    const double t0 = a.time_acc ? a.t_base + (*a.time_acc - dt) : a.t_base;
+    double u2 = 0.0;
+    if (inside) {
+        const long long c = (long long)(i + R) + (long long)a.mx * (j + R) + a.sz * (k + R);
+        T u[3] = {a.uu[0][c], a.uu[1][c], a.uu[2][c]};
+        if (t0 > a.switch_time) {
+            // y = 2 pi xx / (ds ngrid), xx = (global index - nmin) ds
+            const double g[3] = {(double)(a.gx0 + i), (double)(a.gy0 + j), (double)(a.gz0 + k)};
+            const int ng[3] = {a.ngx, a.ngy, a.ngz};
+            double kdy = 0.0;
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                kdy += a.k[d] * (2.0 * M_PI * ((g[d] * a.ds[d]) / (a.ds[d] * (double)ng[d])));
+            double sk, ck, sp, cp;
+            sincos(kdy, &sk, &ck);
+            sincos(a.phase, &sp, &cp);
+            const double re = ck * cp - sk * sp;
+            const double im = ck * sp + sk * cp;
+            const double amp = sqrt(dt) * a.nn;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                u[d] = u[d] + (T)(amp * (a.fre[d] * re - a.fim[d] * im));
+                a.uu[d][c] = u[d];
+            }
+        }
+        u2 = (double)u[0] * (double)u[0] + (double)u[1] * (double)u[1] + (double)u[2] * (double)u[2];
+    }
+    if (a.umax2_next != nullptr)
+        wave_max_to_global(u2, a.umax2_next);
+}
+
+template <typename T>
+void launch_forcing(const ForcingArgs<T>& a, hipStream_t s)
+{
+    const dim3 tpb(64, 4, 1);
+    const dim3 bpg((unsigned)((a.nx + 63) / 64), (unsigned)((a.ny + 3) / 4), (unsigned)a.nz);
+    hipLaunchKernelGGL((forcing_kernel<T>), bpg, tpb, 0, s, a);
+}
+
+template void launch_forcing<AcRealT>(const ForcingArgs<AcRealT>&, hipStream_t);
+
 // ---------------------------------------------------------------- fills
 template <typename T>
 __global__ void fill_kernel(T* dst, size_t n, T v)

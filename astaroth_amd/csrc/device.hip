@@ -73,6 +73,7 @@ struct device_s {
     double cfl_courant;
     int cfl_mode;                 // 0 courant, 1 reference (2 when AC_additive_timestep)
     bool periodic_fused;          // substeps read the periodic images (no halo fill before them)
+    bool forcing_time_acc;        // forcing start time = AC_current_time + time_acc - dt
     double* host_pinned;          // staging for small D2H reads
     // launch shape
     TiledShape shape;
@@ -229,6 +230,58 @@ static AcResult launch_solve(Device d, const SolveArgs<AcReal>& a_in, hipStream_
     }
     return AC_SUCCESS;
 }
+
+// The helical forcing of substep 2 (ref mhdsolver.ac:614-631): added to u in `out` after the
+// solve. With umax2_next set, the forcing kernel reduces max |u|^2 of the forced state instead
+// of the solve.
+static void launch_forcing_step(Device d, const SolveArgs<AcReal>& a, hipStream_t s,
+                                unsigned long long* umax2_next)
+{
+    const AcMeshInfo& info = d->local_config;
+    const AcReal* r = info.real_params;
+    ForcingArgs<AcReal> f;
+    memset((void*)&f, 0, sizeof(f));
+    for (int q = 0; q < 3; ++q)
+        f.uu[q] = d->out[VTXBUF_UUX + q];
+    f.nx = a.nx;
+    f.ny = a.ny;
+    f.nz = a.nz;
+    f.mx = a.mx;
+    f.my = a.my;
+    f.sz = a.sz;
+    const int3 off = info.int3_params[AC_multigpu_offset];
+    const int3 ng = info.int3_params[AC_ngrid];
+    f.gx0 = off.x;
+    f.gy0 = off.y;
+    f.gz0 = off.z;
+    f.ngx = ng.x;
+    f.ngy = ng.y;
+    f.ngz = ng.z;
+    const AcReal3 ds = info.real3_params[AC_ds];
+    f.ds[0] = (double)ds.x;
+    f.ds[1] = (double)ds.y;
+    f.ds[2] = (double)ds.z;
+    f.k[0] = (double)r[AC_k_forcex];
+    f.k[1] = (double)r[AC_k_forcey];
+    f.k[2] = (double)r[AC_k_forcez];
+    f.fre[0] = (double)r[AC_ff_hel_rex];
+    f.fre[1] = (double)r[AC_ff_hel_rey];
+    f.fre[2] = (double)r[AC_ff_hel_rez];
+    f.fim[0] = (double)r[AC_ff_hel_imx];
+    f.fim[1] = (double)r[AC_ff_hel_imy];
+    f.fim[2] = (double)r[AC_ff_hel_imz];
+    f.phase = (double)r[AC_forcing_phase];
+    const double cs = sqrt((double)r[AC_cs2_sound]);
+    f.nn = cs * (double)r[AC_forcing_magnitude] * sqrt((double)r[AC_kaver] * cs);
+    f.t_base = (double)r[AC_current_time];
+    f.switch_time = (double)r[AC_switch_forcing];
+    f.time_acc = d->forcing_time_acc ? a.time_acc : nullptr;
+    f.s = a;
+    f.umax2_next = umax2_next;
+    launch_forcing<AcReal>(f, s);
+}
+
+static bool forcing_on(Device d) { return d->local_config.bool_params[AC_lforcing] != 0; }
 
 extern "C" {
 
@@ -416,6 +469,12 @@ AcResult acDeviceLoadScalarUniform(const Device d, const Stream, const AcRealPar
         return fail("acDeviceLoadScalarUniform: invalid parameter");
     d->local_config.real_params[param] = value;
     return acHostUpdateParams(&d->local_config);
+}
+
+AcResult acDeviceLoadForcingParams(const Device d, const Stream, const ForcingParams fp)
+{
+    loadForcingParamsToMeshInfo(fp, &d->local_config);
+    return AC_SUCCESS;
 }
 
 AcResult acDeviceLoadVectorUniform(const Device d, const Stream, const AcReal3Param param,
@@ -739,7 +798,10 @@ AcResult acDeviceIntegrateSubstep(const Device d, const Stream stream, const int
         return AC_FAILURE;
     SolveArgs<AcReal> a;
     fill_solve_args(d, a, step_number, dt);
-    return launch_solve(d, a, s);
+    const AcResult r = launch_solve(d, a, s);
+    if (r == AC_SUCCESS && step_number == 2 && forcing_on(d))
+        launch_forcing_step(d, a, s, nullptr);
+    return r;
 }
 
 AcResult acDevicePeriodicBoundcondStep(const Device d, const Stream stream,
@@ -922,7 +984,9 @@ AcResult acDeviceIntegrateSubstepDeviceDt(const Device d, const Stream stream,
         launch_zero_bytes(next, 2 * sizeof(unsigned long long), s);
         a.time_acc = d->time_acc;
         if (d->cfl_mode == 0) {
-            a.umax2_next = next;  // max |u_new|^2, reduced by the solve kernel itself
+            // max |u_new|^2, reduced by the solve kernel itself (by the forcing kernel when
+            // forcing follows the solve)
+            a.umax2_next = forcing_on(d) ? nullptr : next;
         }
         else {
             // The reference modes reduce |u| and vA of the state entering this substep: one
@@ -938,6 +1002,8 @@ AcResult acDeviceIntegrateSubstepDeviceDt(const Device d, const Stream stream,
         }
     }
     const AcResult r = launch_solve(d, a, s);
+    if (r == AC_SUCCESS && step_number == 2 && forcing_on(d))
+        launch_forcing_step(d, a, s, d->cfl_mode == 0 ? d->umax2 + 2 * nxt : nullptr);
     if (step_number == 2)
         d->cfl_cur = nxt;
     d->cfl_courant = (double)courant;
@@ -950,8 +1016,9 @@ AcResult acDeviceIntegrateStepsCFL(const Device d, const Stream stream, const in
     AC_HIP_CHECK(hipSetDevice(d->id));
     hipStream_t s = get_stream(d, stream);
     const int3 n = d->local_config.int3_params[AC_nlocal];
-    if (time_out)
-        launch_zero_bytes(d->time_acc, sizeof(double), s);
+    // time_acc accumulates the simulated time of this call (the forcing's step start times are
+    // AC_current_time plus it)
+    launch_zero_bytes(d->time_acc, sizeof(double), s);
     // The tiled solve kernels read the periodic images of the halo cells themselves, so no fill
     // precedes their substeps (launch_solve fills first where another kernel runs). The reference
     // timestep modes reduce over the state entering substep 2 with derivatives that read the
@@ -966,8 +1033,10 @@ AcResult acDeviceIntegrateStepsCFL(const Device d, const Stream stream, const in
             if (!fused)
                 launch_periodic_fill<AcReal>(d->in, NF, n.x, n.y, n.z, s);
             d->periodic_fused = fused;
+            d->forcing_time_acc = true;
             const AcResult r = acDeviceIntegrateSubstepDeviceDt(d, stream, sub, courant);
             d->periodic_fused = false;
+            d->forcing_time_acc = false;
             if (r != AC_SUCCESS)
                 return AC_FAILURE;
             acDeviceSwapBuffers(d);

@@ -34,6 +34,7 @@ const NamedParam kParams[] = {
     {"AC_solve_tile_x", 0, AC_solve_tile_x},
     {"AC_solve_tile_y", 0, AC_solve_tile_y},
     {"AC_solve_zchunk", 0, AC_solve_zchunk},
+    {"AC_forcing_period_steps", 0, AC_forcing_period_steps},
     {"AC_ngrid", 1, AC_ngrid},
     {"AC_nlocal", 1, AC_nlocal},
     {"AC_mlocal", 1, AC_mlocal},
@@ -60,6 +61,23 @@ const NamedParam kParams[] = {
     {"AC_dsmin", 2, AC_dsmin},
     {"AC_current_time", 2, AC_current_time},
     {"AC_dt", 2, AC_dt},
+    {"AC_forcing_magnitude", 2, AC_forcing_magnitude},
+    {"AC_relhel", 2, AC_relhel},
+    {"AC_kmin", 2, AC_kmin},
+    {"AC_kmax", 2, AC_kmax},
+    {"AC_forcing_phase", 2, AC_forcing_phase},
+    {"AC_k_forcex", 2, AC_k_forcex},
+    {"AC_k_forcey", 2, AC_k_forcey},
+    {"AC_k_forcez", 2, AC_k_forcez},
+    {"AC_ff_hel_rex", 2, AC_ff_hel_rex},
+    {"AC_ff_hel_rey", 2, AC_ff_hel_rey},
+    {"AC_ff_hel_rez", 2, AC_ff_hel_rez},
+    {"AC_ff_hel_imx", 2, AC_ff_hel_imx},
+    {"AC_ff_hel_imy", 2, AC_ff_hel_imy},
+    {"AC_ff_hel_imz", 2, AC_ff_hel_imz},
+    {"AC_kaver", 2, AC_kaver},
+    {"AC_switch_forcing", 2, AC_switch_forcing},
+    {"AC_forcing_period_t", 2, AC_forcing_period_t},
     {"AC_ds", 3, AC_ds},
     {"AC_inv_ds", 3, AC_inv_ds},
     {"AC_inv_ds_2", 3, AC_inv_ds_2},
@@ -68,6 +86,7 @@ const NamedParam kParams[] = {
     {"AC_lentropy", 4, AC_lentropy},
     {"AC_additive_timestep", 4, AC_additive_timestep},
     {"AC_periodic_grid", 4, AC_periodic_grid_all},
+    {"AC_lforcing", 4, AC_lforcing},
 };
 
 inline int3 mk3(int x, int y, int z)
@@ -353,6 +372,11 @@ AcResult acSetDefaultConfig(AcMeshInfo* config)
     c.real_params[AC_lnT0] = (AcReal)1.2;
     c.real_params[AC_lnrho0] = (AcReal)1.3;
     c.real_params[AC_K_heatcond] = (AcReal)1e-3;
+    c.int_params[AC_forcing_period_steps] = 1;
+    c.real_params[AC_forcing_magnitude] = (AcReal)1e-5;
+    c.real_params[AC_relhel] = (AcReal)0.0;
+    c.real_params[AC_kmin] = (AcReal)0.8;
+    c.real_params[AC_kmax] = (AcReal)1.2;
     c.bool_params[AC_lupwd] = 1;
     c.bool_params[AC_lentropy] = 1;
     c.bool_params[AC_periodic_grid_all] = 1;

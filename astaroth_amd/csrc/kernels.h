@@ -112,6 +112,29 @@ void launch_cfl_state(const T* const* fields, const Coeffs<T>& C, int nx, int ny
 
 // ---- one-pass diagnostics: per field min, max, sum of squares; then sum, max and min of
 // |u|^2, |curl A|^2 and |curl A|^2 / exp(lnrho)
+// ---- helical forcing (aux_kernels.hip) ----
+// Adds the forcing of ref mhdsolver.ac:286-331 to u (the `out` buffers of the substep-2 solve,
+// interior cells) when the step's start time exceeds switch_forcing; with umax2_next set, also
+// reduces max |u|^2 of the forced state there (the courant timestep mode, in place of the
+// solve's own reduction).
+template <typename T>
+struct ForcingArgs {
+    T* uu[3];
+    int nx, ny, nz, mx, my;
+    long long sz;
+    int gx0, gy0, gz0;               // global index of the first interior cell (0 on one device)
+    int ngx, ngy, ngz;               // AC_ngrid
+    double ds[3];
+    double k[3], fre[3], fim[3], phase;
+    double nn;                       // cs AC_forcing_magnitude sqrt(AC_kaver cs)
+    double t_base, switch_time;      // start time of the step = t_base [+ *time_acc - dt]
+    const double* time_acc;          // device-accumulated time including this step, or null
+    SolveArgs<T> s;                  // dt of the step (solve_dt)
+    unsigned long long* umax2_next;  // or null
+};
+template <typename T>
+void launch_forcing(const ForcingArgs<T>& a, hipStream_t stream);
+
 constexpr int DIAG_SLOTS = 3 * NF + 9;
 
 // ---- periodic fill: every halo cell of every listed field, one launch ----

@@ -24,6 +24,7 @@
 
 #include <stdbool.h>
 #include <stddef.h>
+#include <stdint.h>
 
 #ifdef __cplusplus
 #define AC_BEGIN_C_DECLARATIONS extern "C" {
@@ -93,6 +94,7 @@ typedef enum {
     AC_solve_tile_x,     /* launch shape of the solve kernel (0 = autotune) */
     AC_solve_tile_y,
     AC_solve_zchunk,
+    AC_forcing_period_steps, /* steps between new forcing parameters (config/astaroth.conf:34) */
     NUM_INT_PARAMS
 } AcIntParam;
 
@@ -127,6 +129,24 @@ typedef enum {
     AC_dsmin,
     AC_current_time,
     AC_dt,
+    /* helical forcing (ref mhdsolver.ac:239-257, config/astaroth.conf:34-35,58-61) */
+    AC_forcing_magnitude,
+    AC_relhel,
+    AC_kmin,
+    AC_kmax,
+    AC_forcing_phase,
+    AC_k_forcex,
+    AC_k_forcey,
+    AC_k_forcez,
+    AC_ff_hel_rex,
+    AC_ff_hel_rey,
+    AC_ff_hel_rez,
+    AC_ff_hel_imx,
+    AC_ff_hel_imy,
+    AC_ff_hel_imz,
+    AC_kaver,
+    AC_switch_forcing,
+    AC_forcing_period_t,
     NUM_REAL_PARAMS
 } AcRealParam;
 
@@ -143,6 +163,7 @@ typedef enum {
     AC_lentropy,         /* ref hostdefine LENTROPY (mhdsolver.ac:99): evolve VTXBUF_ENTROPY */
     AC_additive_timestep,
     AC_periodic_grid_all,
+    AC_lforcing,         /* ref mhdsolver.ac:533,622: add the helical forcing to u at substep 2 */
     NUM_BOOL_PARAMS
 } AcBoolParam;
 
@@ -517,6 +538,46 @@ AcResult acHostAppendSnapshotInfo(const char* dir, const int step, const int mod
                                   const double t_step, const AcMeshInfo* info);
 /* The last row of dir/snapshots_info.csv (the snapshot a restart loads, ref main.cc:579-605). */
 AcResult acHostReadSnapshotInfo(const char* dir, int* step, int* modstep, double* t_step);
+
+/* ---- Helical forcing (ref include/astaroth_forcing.h, include/astaroth_random.h) ----
+ * With AC_lforcing set, the last substep of every RK3 step adds to u the forcing of
+ * ref mhdsolver.ac:286-331,614-631 (while AC_current_time > AC_switch_forcing):
+ *   f(x) = sqrt(dt) cs AC_forcing_magnitude sqrt(AC_kaver cs)
+ *          (ff_re cos(k.y + phase) - ff_im sin(k.y + phase)),  y = 2 pi (x - x0) / (ds n),
+ * with k = (AC_k_forcex, ...), ff_re = (AC_ff_hel_rex, ...), ff_im = (AC_ff_hel_imx, ...). The
+ * parameters come from generateHelicalForcingParams and reach a device through
+ * loadForcingParamsToMeshInfo + acDeviceLoadMeshInfo or acDeviceLoadForcingParams. */
+typedef struct {
+    AcReal magnitude;
+    AcReal3 k_force;
+    AcReal3 ff_hel_re;
+    AcReal3 ff_hel_im;
+    AcReal phase;
+    AcReal kaver;
+} ForcingParams;
+
+/* Seeds the process-wide std::mt19937 the forcing generator draws from (ref
+ * src/utils/astaroth_random.cc:seed_rng; initial seed 19937123 as there). */
+void seed_rng(uint32_t seed);
+/* One uniform [0, 1) draw from that generator (ref astaroth_random.cc:random_uniform_real_01). */
+AcReal random_uniform_real_01(void);
+/* A random helical forcing (ref src/utils/helical_forcing.cc:generateHelicalForcingParams):
+ * integer k with kmin <= |k| <= kmax, drawn uniformly from that shell; a random unit e not
+ * parallel to k; f_re = |k| (k x e) / D and f_im = relhel k x (k x e) / D with
+ * D = sqrt(1 + relhel^2) |k|^2 sqrt(1 - (k.e)^2/|k|^2), so |f_re|^2 + |f_im|^2 = 1; phase uniform
+ * in [-pi, pi); kaver = (kmax - kmin) / 2 + kmin. Same draws, in the same order, from the same
+ * generator as the reference, so a seeded sequence of parameters matches it. */
+ForcingParams generateHelicalForcingParams(const AcReal relhel, const AcReal magnitude,
+                                           const AcReal kmin, const AcReal kmax);
+/* Copies the parameters into the forcing fields of *info (ref
+ * src/utils/astaroth_forcing.cc:loadForcingParamsToMeshInfo). Returns 0. */
+int loadForcingParamsToMeshInfo(const ForcingParams forcing_params, AcMeshInfo* info);
+/* Prints the parameters to stdout (ref astaroth_forcing.cc:printForcingParams). */
+void printForcingParams(const ForcingParams forcing_params);
+/* The forcing fields of the device's configuration, as the reference's driver loads them with
+ * acGridLoadScalarUniform (ref samples/standalone_mpi/host_forcing.cc:loadForcingParamsToGrid). */
+AcResult acDeviceLoadForcingParams(const Device device, const Stream stream,
+                                   const ForcingParams forcing_params);
 
 /* Library identification: sizeof(AcReal), sizeof(AcMeshInfo), sizeof(AcMesh) of this build and
  * the offload target it was built for (a caller checks its own struct sizes against these, as

@@ -90,6 +90,13 @@ def lib():
         L.oracle_reduce_f32.restype = ctypes.c_longdouble
         L.refcpu_integrate_step_f64.argtypes = [pp_d, pp_d, cfg, ctypes.c_double, ctypes.c_int]
         L.oracle_num_threads.restype = ctypes.c_int
+        L.oracle_seed_rng.argtypes = [ctypes.c_uint32]
+        L.oracle_seed_rng.restype = None
+        L.oracle_mt19937_draw.restype = ctypes.c_uint32
+        L.oracle_shell_size.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.oracle_helical_forcing_params.argtypes = [ctypes.c_double] * 3 + [P(ctypes.c_double)]
+        L.oracle_add_forcing_f64.argtypes = [pp_d] + [ctypes.c_int] * 3 + [P(ctypes.c_double)]
+        L.oracle_add_forcing_f32.argtypes = [pp_f] + [ctypes.c_int] * 3 + [P(ctypes.c_double)]
         _lib = L
     return _lib
 
@@ -143,6 +150,48 @@ def refcpu_step(fields, wbuf, cfg, dt, nthreads=0):
 
 def num_threads():
     return lib().oracle_num_threads()
+
+
+# ---------------------------------------------------------------- helical forcing
+def seed_rng(seed):
+    """seed_rng of ref src/utils/astaroth_random.cc on the oracle's own std::mt19937."""
+    lib().oracle_seed_rng(seed)
+
+
+def mt19937_draw():
+    return int(lib().oracle_mt19937_draw())
+
+
+def shell_size(kmin, kmax):
+    return int(lib().oracle_shell_size(kmin, kmax))
+
+
+def helical_forcing_params(relhel, kmin, kmax):
+    """generateHelicalForcingParams (ref src/utils/helical_forcing.cc:325-341) in FP64:
+    dict(k, f_re, f_im (3-vectors), phase, kaver), before rounding to AcReal."""
+    out = (ctypes.c_double * 11)()
+    if lib().oracle_helical_forcing_params(relhel, kmin, kmax, out) != 0:
+        raise ValueError("empty wave-vector shell")
+    v = list(out)
+    return dict(k=np.array(v[0:3]), f_re=np.array(v[3:6]), f_im=np.array(v[6:9]), phase=v[9],
+                kaver=v[10])
+
+
+def add_forcing(fields, cfg, k, f_re, f_im, phase, magnitude, kaver, cs2_sound, ngrid, dt,
+                current_time, switch_forcing):
+    """The substep-2 forcing of ref mhdsolver.ac:286-331,614-631 added to fields[1:4] in place
+    (long double, rounded to the fields' dtype). `cfg` supplies the interior size and ds."""
+    p = (ctypes.c_double * 22)(*(list(map(float, k)) + list(map(float, f_re)) +
+                                 list(map(float, f_im)) +
+                                 [float(phase), float(magnitude), float(kaver), float(cs2_sound)] +
+                                 [cfg.ds[0], cfg.ds[1], cfg.ds[2]] + list(map(float, ngrid)) +
+                                 [float(dt), float(current_time), float(switch_forcing)]))
+    uu = np.ascontiguousarray(fields[1:4])
+    if fields.dtype == np.float64:
+        lib().oracle_add_forcing_f64(_ptrs(uu), cfg.nx, cfg.ny, cfg.nz, p)
+    else:
+        lib().oracle_add_forcing_f32(_ptrs(uu), cfg.nx, cfg.ny, cfg.nz, p)
+    fields[1:4] = uu
 
 
 # ---------------------------------------------------------------- verification metric
