@@ -35,10 +35,21 @@ def append_row(lib, path, step, t_step, dt, diag):
     return bool(nan.value)
 
 
-def run_timeseries(dev, nsteps, every, path, courant=0.2):
+def run_timeseries(dev, nsteps, every, path, courant=0.2, forcing_seed=312256655):
     """Integrates nsteps RK3 steps with the device CFL timestep, appending a diagnostics row
-    at step 0 and every `every` steps. Returns (steps done, simulated time)."""
+    at step 0 and every `every` steps. Returns (steps done, simulated time).
+
+    With AC_lforcing set in the device's configuration, new helical forcing parameters are
+    drawn before every AC_forcing_period_steps-th step and the step's start time is loaded as
+    AC_current_time, as the reference driver's GenerateForcing action does (ref
+    samples/standalone_mpi/main.cc:1377-1383,1447-1453,1567-1572; generator seeded once with
+    `forcing_seed`, the driver's seed at main.cc:1302)."""
     lib = dev.lib
+    info = dev.info
+    forced = bool(info["AC_lforcing"])
+    period = max(1, int(info["AC_forcing_period_steps"])) if forced else 0
+    if forced:
+        lib.seed_rng(forcing_seed)
     ac.check(lib.acDeviceUpdateTimestepCFL(dev.handle, ac.STREAM_DEFAULT, courant), "CFL")
     t = ctypes.c_double(0.0)
     step = 0
@@ -49,6 +60,20 @@ def run_timeseries(dev, nsteps, every, path, courant=0.2):
         if append_row(lib, path, step, t.value, dt.value, dev.diagnostics()) or step >= nsteps:
             return step, t.value
         k = min(every, nsteps - step)
-        ac.check(lib.acDeviceIntegrateStepsCFL(dev.handle, ac.STREAM_DEFAULT, k, courant,
-                                               ctypes.byref(t)), "acDeviceIntegrateStepsCFL")
-        step += k
+        while k > 0:
+            chunk = k
+            if forced:
+                if step % period == 0:
+                    fp = lib.generateHelicalForcingParams(info["AC_relhel"],
+                                                          info["AC_forcing_magnitude"],
+                                                          info["AC_kmin"], info["AC_kmax"])
+                    ac.check(lib.acDeviceLoadForcingParams(dev.handle, ac.STREAM_DEFAULT, fp),
+                             "acDeviceLoadForcingParams")
+                ac.check(lib.acDeviceLoadScalarUniform(dev.handle, ac.STREAM_DEFAULT,
+                                                       ac.AC_current_time, lib.np_real(t.value)),
+                         "AC_current_time")
+                chunk = min(k, period - step % period)
+            ac.check(lib.acDeviceIntegrateStepsCFL(dev.handle, ac.STREAM_DEFAULT, chunk, courant,
+                                                   ctypes.byref(t)), "acDeviceIntegrateStepsCFL")
+            step += chunk
+            k -= chunk
