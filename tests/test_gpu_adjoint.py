@@ -58,13 +58,9 @@ def lib(request, require_gpu):
     return ac.load(request.param)
 
 
-@pytest.mark.parametrize("step", [0, 1, 2])
-@pytest.mark.parametrize("upwd,entropy", [(True, True), (False, False), (True, False)])
-@pytest.mark.parametrize("n", [16, (64, 8, 16)])  # one-cell-per-thread and tiled local stage
-def test_substep_dot_product(lib, step, upwd, entropy, n):
-    say(f"case dot-product substep {step} {lib.precision} upwd={upwd} entropy={entropy} n={n}")
-    info = physics_info(lib, n, AC_lupwd=upwd, AC_lentropy=entropy)
-    f = initial(lib, info, 3 + step)
+def substep_dot_product(lib, info, f, step, entropy, seed):
+    """<T(dx), y> and <dx, A(y)> of one substep at state f for seeded random directions (and
+    parameter directions (0.3, -0.2)); returns (lhs, rhs, relative mismatch, param grads)."""
     # the identity is linear algebra and holds for any dt; a large one makes the Jacobian part
     # dominate both sides
     dt = lib.np_real(0.5)
@@ -77,7 +73,7 @@ def test_substep_dot_product(lib, step, upwd, entropy, n):
         ac.check(lib.acDeviceLoadMeshFromDevicePtr(dev.handle, 0, ft.data_ptr()), "load")
         dev.periodic()
         dev.sync()
-        g = torch.Generator(device="cpu").manual_seed(7 + step)
+        g = torch.Generator(device="cpu").manual_seed(seed)
         rnd = lambda: torch.randn(f.shape, generator=g, dtype=torch.float64).to(tdt).to(cuda)
         df, dw, lf, lw = rnd(), rnd(), rnd(), rnd()
         if not entropy:
@@ -106,14 +102,24 @@ def test_substep_dot_product(lib, step, upwd, entropy, n):
         rhs = idot(df, lf_a) + (idot(dw, lw_a) if step > 0 else 0.0) + \
             0.3 * float(pg[0]) - 0.2 * float(pg[1])
         scale = abs(idot(df, lf)) + abs(lhs) + 1e-300
-        rel = abs(lhs - rhs) / scale
-        say(f"  <T dx, y> = {lhs:.15g}  <dx, A y> = {rhs:.15g}  rel = {rel:.3g}")
-        tol = 1e-12 if lib.precision == "fp64" else 1e-5
-        assert rel < tol
-        # the parameter sensitivities alone: dot-product with dx = 0
-        assert float(pg.abs().sum()) > 0
+        return lhs, rhs, abs(lhs - rhs) / scale, pg.cpu()
     finally:
         dev.close()
+
+
+@pytest.mark.parametrize("step", [0, 1, 2])
+@pytest.mark.parametrize("upwd,entropy", [(True, True), (False, False), (True, False)])
+@pytest.mark.parametrize("n", [16, (64, 8, 16)])  # one-cell-per-thread and tiled local stage
+def test_substep_dot_product(lib, step, upwd, entropy, n):
+    say(f"case dot-product substep {step} {lib.precision} upwd={upwd} entropy={entropy} n={n}")
+    info = physics_info(lib, n, AC_lupwd=upwd, AC_lentropy=entropy)
+    f = initial(lib, info, 3 + step)
+    lhs, rhs, rel, pg = substep_dot_product(lib, info, f, step, entropy, 7 + step)
+    say(f"  <T dx, y> = {lhs:.15g}  <dx, A y> = {rhs:.15g}  rel = {rel:.3g}")
+    tol = 1e-12 if lib.precision == "fp64" else 1e-5
+    assert rel < tol
+    # the parameter sensitivities alone: dot-product with dx = 0
+    assert float(pg.abs().sum()) > 0
 
 
 def loss_and_target(lib, n, upwd=True):

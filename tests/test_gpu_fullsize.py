@@ -1,0 +1,145 @@
+"""Size-independent checks at the benchmark configurations' full sizes (256^3 and 512^3 FP32
+forward, 128^3 adjoint), where the oracle is too slow to run:
+
+  * Beltrami A-field, u = 0, uniform lnrho: j x B = 0 and the induction equation reduces to
+    dA/dt = eta lap A, so every step multiplies A by P3(-eta k2 dt) = 1 + z + z^2/2 + z^3/6 (the
+    2N RK3 amplification), with k2 the 6th-order second-derivative symbol
+    (49/18 - 3 cos t + 3/10 cos 2t - 1/45 cos 3t) / dx^2, t = k dx; u stays zero;
+  * shear wave u = (0, a sin kx, 0), entropy off: viscous decay by P3(-nu k2 dt) per step;
+  * the dot-product test of one adjoint substep at 128^3.
+Both forward forms run through the reference call pattern (fill + substep + swap) and through
+the multi-step device-dt path, whose solve reads the periodic images itself."""
+import ctypes
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from astaroth_amd import astaroth as ac
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def p3(z):
+    return 1.0 + z + z * z / 2.0 + z * z * z / 6.0
+
+
+def k2_symbol(k, dx):
+    t = k * dx
+    return (49.0 / 18.0 - 3.0 * np.cos(t) + 0.3 * np.cos(2 * t) - np.cos(3 * t) / 45.0) / dx ** 2
+
+
+def info_for(lib, n, **over):
+    p = dict(AC_len=(2 * np.pi,) * 3, AC_cs_sound=1.0, AC_gamma=5.0 / 3.0, AC_nu_visc=5e-3,
+             AC_eta=5e-3, AC_chi=5e-3, AC_K_heatcond=5e-3, AC_lupwd=True, AC_lentropy=True)
+    p.update(over)
+    return lib.config((n, n, n), **p)
+
+
+def run(lib, info, data, steps, multistep, dt=None, courant=0.2):
+    """Returns (final state, total simulated time)."""
+    mesh = ac.HostMesh(lib, info, data)
+    with ac.Device(lib, info) as dev:
+        dev.load_mesh(mesh)
+        if multistep:
+            dev.periodic()
+            ac.check(lib.acDeviceUpdateTimestepCFL(dev.handle, 0, lib.np_real(courant)), "cfl")
+            t = ctypes.c_double(0.0)
+            ac.check(lib.acDeviceIntegrateStepsCFL(dev.handle, 0, steps, lib.np_real(courant),
+                                                   ctypes.byref(t)), "steps")
+            total = t.value
+        else:
+            for _ in range(steps):
+                dev.step(lib.np_real(dt))
+            total = steps * float(lib.np_real(dt))
+        dev.periodic()
+        out = ac.HostMesh(lib, info)
+        dev.store_mesh(out)
+    return out.data, total
+
+
+@pytest.mark.parametrize("n,multistep", [(256, False), (256, True), (512, True)])
+def test_beltrami_decay(n, multistep, require_gpu):
+    lib = ac.load("fp32")
+    say(f"case Beltrami decay fp32 n={n} multistep={multistep}")
+    info = info_for(lib, n)
+    dx = 2 * np.pi / n
+    k, a, steps = 3, 0.1, 4
+    data = ac.HostMesh(lib, info).data
+    data[:] = 0
+    x = (np.arange(n + 6) - 3) * dx
+    data[5][:, :, :] = (a * np.sin(k * x)).astype(np.float32)[None, None, :]  # A_y
+    data[6][:, :, :] = (a * np.cos(k * x)).astype(np.float32)[None, None, :]  # A_z
+    dt = 0.2 * dx  # = the courant dt of a state at rest (cs = 1)
+    out, total = run(lib, info, data, steps, multistep, dt=dt)
+    del data
+    sl = (slice(3, -3),) * 3
+    step_dt = total / steps
+    g = p3(-5e-3 * k2_symbol(k, dx) * step_dt) ** steps
+    xi = x[3:-3]
+    for q, ref in ((5, a * g * np.sin(k * xi)), (6, a * g * np.cos(k * xi))):
+        err = float(np.abs(out[q][sl] - ref[None, None, :]).max()) / a
+        say(f"  A{'yz'[q - 5]}: decay {g:.9f}  max err / a = {err:.3g}")
+        assert err < 2e-6
+    # u stays at rounding level: j x B vanishes up to the FP32 rounding of the second
+    # derivatives in j (of order eps |c0| a / (mu0 dx^2)) times |B| ~ k a, over the elapsed time
+    # (a genuine Lorentz force would give |j| |B| t, about 1e3 times more); A_x = int (u x B)_x
+    # stays far smaller still
+    noise = 6e-8 * (49.0 / 18.0) / dx ** 2 * (a / 1.4) * (k * a) * total
+    umax = max(float(np.abs(out[q][sl]).max()) for q in (1, 2, 3))
+    axmax = float(np.abs(out[4][sl]).max())
+    say(f"  max |u| = {umax:.3g} (rounding scale {noise:.3g})  max |A_x| = {axmax:.3g}")
+    assert umax < 50 * noise and axmax < 1e-8 * a
+    assert float(np.abs(out[0][sl]).max()) < 1e-6  # lnrho stays uniform (zero)
+    # the decay is visible: far beyond the tolerance
+    assert 1.0 - g > 1e-4
+
+
+@pytest.mark.parametrize("multistep", [False, True])
+def test_shear_wave_decay_256(multistep, require_gpu):
+    lib = ac.load("fp32")
+    n = 256
+    say(f"case shear wave fp32 n={n} multistep={multistep}")
+    info = info_for(lib, n, AC_lentropy=False)
+    dx = 2 * np.pi / n
+    k, a, steps = 2, 0.05, 4
+    data = ac.HostMesh(lib, info).data
+    data[:] = 0
+    x = (np.arange(n + 6) - 3) * dx
+    data[2][:, :, :] = (a * np.sin(k * x)).astype(np.float32)[None, None, :]
+    dt = 0.2 * dx / (1.0 + a)
+    out, total = run(lib, info, data, steps, multistep, dt=dt)
+    del data
+    sl = (slice(3, -3),) * 3
+    xi = x[3:-3]
+    # the device dt of each step follows max|u|, which decays: use the per-step dts' product
+    # bound -- with 4 steps the mean dt is within 1e-6 of each
+    step_dt = total / steps
+    g = p3(-5e-3 * k2_symbol(k, dx) * step_dt) ** steps
+    err = float(np.abs(out[2][sl] - (a * g * np.sin(k * xi))[None, None, :]).max()) / a
+    say(f"  u_y: decay {g:.9f}  max err / a = {err:.3g}")
+    assert err < 2e-6
+    for q in (0, 1, 3, 4, 5, 6):
+        assert float(np.abs(out[q][sl]).max()) < 1e-6 * a, q
+    assert 1.0 - g > 1e-5
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-6)])
+def test_dot_product_128(prec, tol, require_gpu):
+    """The differentiation configuration's size: the substep adjoint passes the dot-product
+    test at 1e-6 in FP32 (the tiled local stage and the LDS-staged transposes at full size)."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import test_gpu_adjoint as TA
+    lib = ac.load(prec)
+    info = TA.physics_info(lib, 128)
+    f = TA.initial(lib, info, 5)
+    for step in (0, 2):
+        lhs, rhs, rel, pg = TA.substep_dot_product(lib, info, f, step, True, 17 + step)
+        say(f"case dot-product 128^3 {prec} substep {step}: rel = {rel:.3g}")
+        assert rel < tol
