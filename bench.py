@@ -256,7 +256,12 @@ def form_precision(ac, n=512, steps=200, mid=100, seed=512, kbs=(8, 16, 32), nmo
     and Kahan-compensated FP32 (compensation stored in 8, 16 or 32 bits). This initial condition
     is magnetically dominated (Alfven speed well above cs + |u|), so dt is recomputed every step
     from the FP64 state as 0.2 dx / (cs + max|u| + max v_A) and the same dt sequence is replayed
-    in every FP32 variant. Errors at `mid` and `steps` steps."""
+    in every FP32 variant. Errors at `mid` and `steps` steps.
+
+    Inputs are identical: the FP64 reference starts from the FP32-rounded initial condition the
+    FP32 runs start from, so the errors measure the FP32 arithmetic and storage. A second FP64
+    run from the unrounded condition measures what the rounding of the inputs alone grows into
+    ("input_rounding")."""
     torch = _torch()
     n3 = n ** 3
     dx = 2 * np.pi / n
@@ -264,6 +269,13 @@ def form_precision(ac, n=512, steps=200, mid=100, seed=512, kbs=(8, 16, 32), nmo
     info64 = l64.config((n, n, n), **PHYS)
     dev = ac.Device(l64, info64)
     dev.init_fourier(seed, nmodes, kmax, ampl, perturb)
+    exact_ic = state_tensor(l64, dev)
+    rounded_ic = exact_ic.float().double().contiguous()
+    torch.cuda.synchronize()
+    if l64.acDeviceLoadMeshFromDevicePtr(dev.handle, 0, ctypes.c_void_p(rounded_ic.data_ptr())):
+        raise RuntimeError("acDeviceLoadMeshFromDevicePtr failed")
+    dev.sync()
+    del rounded_ic
     dts, vmax = [], []
     t_ref = 0.0
     refs = {}
@@ -282,6 +294,23 @@ def form_precision(ac, n=512, steps=200, mid=100, seed=512, kbs=(8, 16, 32), nmo
             refs[k + 1] = state_tensor(l64, dev)
     dev.close()
     runs = {"fp64": {"cell_updates_per_s": n3 * steps / t_ref}}
+    # the FP64 run from the unrounded initial condition, same dt sequence
+    dev = ac.Device(l64, info64)
+    torch.cuda.synchronize()
+    if l64.acDeviceLoadMeshFromDevicePtr(dev.handle, 0, ctypes.c_void_p(exact_ic.data_ptr())):
+        raise RuntimeError("acDeviceLoadMeshFromDevicePtr failed")
+    dev.sync()
+    del exact_ic
+    rec = {}
+    for k in range(steps):
+        dev.step(l64.np_real(dts[k]))
+        if k + 1 in refs:
+            e = field_errors_gpu(refs[k + 1], state_tensor(l64, dev))
+            rec[f"errors_after_{k + 1}"] = {ac.FIELD_NAMES[q]: e[q] for q in range(NF)}
+            rec[f"max_maxrel_after_{k + 1}"] = max(x["maxrel"] for x in e)
+    dev.close()
+    torch.cuda.empty_cache()
+    runs["fp64_unrounded_ic"] = rec
     for kb in (0,) + tuple(kbs):
         l32 = ac.load("fp32")
         dv = ac.Device(l32, l32.config((n, n, n), AC_kahan_bits=kb, **PHYS))
@@ -310,7 +339,9 @@ def form_precision(ac, n=512, steps=200, mid=100, seed=512, kbs=(8, 16, 32), nmo
             "dt_rule": "0.2 dx / (cs + max|u|" + (" + max vA)" if with_va else ")") +
                        " of the FP64 state, every step, replayed in the FP32 runs",
             "timing": "wall time of the steps only, device-synchronised per step",
-            "reference": "FP64 GPU mode", "runs": runs}
+            "reference": "FP64 GPU mode from the FP32-rounded initial condition (identical "
+                         "inputs); fp64_unrounded_ic: FP64 from the unrounded one",
+            "runs": runs}
 
 
 def form_scale(ac, n=1024, steps=50, kb=16, seed=1024):
