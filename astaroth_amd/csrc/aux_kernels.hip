@@ -472,6 +472,32 @@ void launch_cfl_state(const T* const* fields, const Coeffs<T>& C, int nx, int ny
 template void launch_cfl_state<AcRealT>(const AcRealT* const*, const Coeffs<AcRealT>&, int, int, int,
                                          int, double, unsigned long long*, hipStream_t);
 
+// ---------------------------------------------------------------- compensation records
+template <typename E>
+__global__ void zero_comp_field_kernel(E* first, size_t count)
+{
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < count;
+         t += (size_t)gridDim.x * blockDim.x)
+        first[t * NF] = E(0);
+}
+
+void launch_zero_comp_field(void* slab, size_t count, int kahan_bits, int q, hipStream_t s)
+{
+    size_t nb = (count + 255) / 256;
+    if (nb > 65536)
+        nb = 65536;
+    char* first = (char*)slab + (size_t)q * (kahan_bits / 8);
+    if (kahan_bits == 8)
+        hipLaunchKernelGGL(zero_comp_field_kernel<int8_t>, dim3((unsigned)nb), dim3(256), 0, s,
+                           (int8_t*)first, count);
+    else if (kahan_bits == 16)
+        hipLaunchKernelGGL(zero_comp_field_kernel<int16_t>, dim3((unsigned)nb), dim3(256), 0, s,
+                           (int16_t*)first, count);
+    else if (kahan_bits == 32)
+        hipLaunchKernelGGL(zero_comp_field_kernel<float>, dim3((unsigned)nb), dim3(256), 0, s,
+                           (float*)first, count);
+}
+
 // ---------------------------------------------------------------- helical forcing
 // ref mhdsolver.ac:286-331 (helical_forcing, forcing) and :614-631 (added to u after the final
 // update of substep 2). The geometry and the mode are evaluated in FP64, the force is rounded to

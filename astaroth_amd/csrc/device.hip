@@ -348,7 +348,8 @@ AcResult acDeviceCreate(const int id, const AcMeshInfo device_config, Device* de
         d->in[q] = (AcReal*)d->slab_in + (size_t)q * d->count;
         d->out[q] = (AcReal*)d->slab_out + (size_t)q * d->count;
         d->w[q] = (AcReal*)d->slab_w + (size_t)q * d->count;
-        d->comp[q] = kb > 0 ? (void*)((char*)d->slab_comp + (size_t)q * d->count * (kb / 8)) : nullptr;
+        // one record of NF compensations per cell: field q's are every NF-th, from byte q kb / 8
+        d->comp[q] = kb > 0 ? (void*)((char*)d->slab_comp + (size_t)q * (kb / 8)) : nullptr;
     }
     launch_zero_bytes(d->slab_in, bytes, s0);
     launch_zero_bytes(d->slab_out, bytes, s0);
@@ -600,7 +601,7 @@ AcResult acDeviceLoadVertexBufferWithOffset(const Device d, const Stream stream,
     AC_HIP_CHECK(hipMemcpyAsync(d->in[h] + di, host_mesh.vertex_buffer[h] + si,
                                 sizeof(AcReal) * (size_t)num_vertices, hipMemcpyHostToDevice, s));
     if (d->kahan_bits > 0)
-        launch_zero_bytes(d->comp[h], d->count * (size_t)(d->kahan_bits / 8), s);
+        launch_zero_comp_field(d->slab_comp, d->count, d->kahan_bits, (int)h, s);
     return AC_SUCCESS;
 }
 
@@ -689,7 +690,7 @@ AcResult acDeviceInitFourier(const Device d, const Stream stream, const unsigned
     for (int q = 0; q < NF; ++q) {
         launch_fill<AcReal>(d->out[q], d->count, (AcReal)0, s);
         if (d->kahan_bits > 0)
-            launch_zero_bytes(d->comp[q], d->count * (size_t)(d->kahan_bits / 8), s);
+            launch_zero_comp_field(d->slab_comp, d->count, d->kahan_bits, q, s);
     }
     AC_HIP_CHECK(hipGetLastError());
     // the tables are consumed by the kernel above; free them once it has run
@@ -709,7 +710,7 @@ AcResult acDeviceSetVertexBuffer(const Device d, const Stream stream, const Vert
     launch_fill<AcReal>(d->in[h], d->count, value, s);
     launch_fill<AcReal>(d->out[h], d->count, value, s);
     if (d->kahan_bits > 0)
-        launch_zero_bytes(d->comp[h], d->count * (size_t)(d->kahan_bits / 8), s);
+        launch_zero_comp_field(d->slab_comp, d->count, d->kahan_bits, (int)h, s);
     AC_HIP_CHECK(hipGetLastError());
     return AC_SUCCESS;
 }

@@ -112,6 +112,10 @@ void launch_cfl_state(const T* const* fields, const Coeffs<T>& C, int nx, int ny
 
 // ---- one-pass diagnostics: per field min, max, sum of squares; then sum, max and min of
 // |u|^2, |curl A|^2 and |curl A|^2 / exp(lnrho)
+// Zeroes field q's entries of the per-cell compensation records (count cells, NF entries of
+// kahan_bits / 8 bytes each).
+void launch_zero_comp_field(void* slab, size_t count, int kahan_bits, int q, hipStream_t stream);
+
 // ---- helical forcing (aux_kernels.hip) ----
 // Adds the forcing of ref mhdsolver.ac:286-331 to u (the `out` buffers of the substep-2 solve,
 // interior cells) when the step's start time exceeds switch_forcing; with umax2_next set, also

@@ -228,18 +228,22 @@ __device__ __forceinline__ int ulp_exp_f32(float f)
     return (int)(e > 0u ? e : 1u) - 150;  // ulp(f) = 2^ulp_exp
 }
 
+// The compensation array holds one record per cell: the NF fields' compensations of the cell
+// side by side (NF * KB / 8 bytes), so the tiled kernels move a cell's record in one or two
+// vector accesses. Field q's value of cell c is element c * NF of the array that starts at
+// byte q * KB / 8 (`base` below).
 template <int KB>
 __device__ __forceinline__ float comp_load(const void* base, long long c, float f)
 {
     if (KB == 32) {
-        return ((const float*)base)[c];
+        return ((const float*)base)[c * NF];
     }
     else if (KB == 16) {
-        const int q = ((const int16_t*)base)[c];
+        const int q = ((const int16_t*)base)[c * NF];
         return ldexpf((float)q, ulp_exp_f32(f) - 16);
     }
     else {
-        const int q = ((const int8_t*)base)[c];
+        const int q = ((const int8_t*)base)[c * NF];
         return ldexpf((float)q, ulp_exp_f32(f) - 8);
     }
 }
@@ -248,17 +252,17 @@ template <int KB>
 __device__ __forceinline__ void comp_store(void* base, long long c, float comp, float f)
 {
     if (KB == 32) {
-        ((float*)base)[c] = comp;
+        ((float*)base)[c * NF] = comp;
     }
     else if (KB == 16) {
         float q = rintf(ldexpf(comp, 16 - ulp_exp_f32(f)));
         q = fminf(fmaxf(q, -32767.f), 32767.f);
-        ((int16_t*)base)[c] = (int16_t)q;
+        ((int16_t*)base)[c * NF] = (int16_t)q;
     }
     else {
         float q = rintf(ldexpf(comp, 8 - ulp_exp_f32(f)));
         q = fminf(fmaxf(q, -127.f), 127.f);
-        ((int8_t*)base)[c] = (int8_t)q;
+        ((int8_t*)base)[c * NF] = (int8_t)q;
     }
 }
 
@@ -294,35 +298,6 @@ __device__ __forceinline__ void rk_update(const SolveArgs<T>& a, int q, long lon
     }
 }
 
-// The same update addressed as (uniform plane base, 32-bit in-plane offset): wp, cp and op
-// point at the cell's z plane of w, compensation and output, so every access can use a
-// scalar base register and one vector offset.
-template <typename T, int KB>
-__device__ __forceinline__ T rk_update_plane(T* __restrict__ wp, void* __restrict__ cp,
-                                             T* __restrict__ op, unsigned off, int step, T alpha,
-                                             T beta, T f, T rhs, T dt)
-{
-    T wn = dt * rhs;
-    if (step > 0)
-        wn = alpha * wp[off] + wn;
-    if (step < 2)
-        wp[off] = wn;
-    T fn;
-    if (KB == 0) {
-        fn = add_rn(f, mul_rn(beta, wn));
-    }
-    else {
-        const float cc = comp_load<KB>(cp, off, f);
-        const float y = beta * wn - cc;
-        const float t = f + y;
-        const float cn = (t - f) - y;
-        comp_store<KB>(cp, off, cn, t);
-        fn = t;
-    }
-    op[off] = fn;
-    return fn;
-}
-
 // Access at (uniform base, 32-bit byte offset): the form global loads and stores take as one
 // scalar base register pair plus one vector offset, with no per-lane 64-bit address math.
 template <typename T>
@@ -336,34 +311,87 @@ __device__ __forceinline__ void st_b(T* base, unsigned boff, T v)
     *(T*)((char*)base + boff) = v;
 }
 
-// Raw stored compensation of one value (float bits, or the sign-extended int8/int16);
-// cboff is the byte offset in the compensation array.
+// A cell's compensation record in registers: NF * KB / 32 dwords (2, 4 or 8).
 template <int KB>
-__device__ __forceinline__ int comp_raw_load(const void* base, unsigned cboff)
+struct CompRec {
+    uint32_t w[KB > 0 ? NF * KB / 32 : 1];
+};
+template <int KB>
+__device__ __forceinline__ CompRec<KB> comp_rec_load(const char* base, unsigned boff)
 {
-    const char* p = (const char*)base + cboff;
-    if (KB == 32)
-        return __float_as_int(*(const float*)p);
-    if (KB == 16)
-        return *(const int16_t*)p;
-    return *(const int8_t*)p;
+    CompRec<KB> r;
+    const char* p = base + boff;
+    if constexpr (KB == 8) {
+        const uint2 v = *(const uint2*)p;
+        r.w[0] = v.x;
+        r.w[1] = v.y;
+    }
+    else if constexpr (KB >= 16) {
+#pragma unroll
+        for (int h = 0; h < KB / 16; ++h) {
+            const uint4 v = *(const uint4*)(p + 16 * h);
+            r.w[4 * h + 0] = v.x;
+            r.w[4 * h + 1] = v.y;
+            r.w[4 * h + 2] = v.z;
+            r.w[4 * h + 3] = v.w;
+        }
+    }
+    return r;
 }
 template <int KB>
-__device__ __forceinline__ void comp_store_b(void* base, unsigned cboff, float comp, float f)
+__device__ __forceinline__ void comp_rec_store(char* base, unsigned boff, const CompRec<KB>& r)
 {
-    char* p = (char*)base + cboff;
-    if (KB == 32) {
-        *(float*)p = comp;
+    char* p = base + boff;
+    if constexpr (KB == 8) {
+        *(uint2*)p = make_uint2(r.w[0], r.w[1]);
     }
-    else if (KB == 16) {
-        float q = rintf(ldexpf(comp, 16 - ulp_exp_f32(f)));
-        q = fminf(fmaxf(q, -32767.f), 32767.f);
-        *(int16_t*)p = (int16_t)q;
+    else if constexpr (KB >= 16) {
+#pragma unroll
+        for (int h = 0; h < KB / 16; ++h)
+            *(uint4*)(p + 16 * h) =
+                make_uint4(r.w[4 * h + 0], r.w[4 * h + 1], r.w[4 * h + 2], r.w[4 * h + 3]);
+    }
+}
+// Field f's raw compensation (float bits, or the sign-extended int8 / int16).
+template <int KB>
+__device__ __forceinline__ int comp_rec_get(const CompRec<KB>& r, int f)
+{
+    if constexpr (KB == 32)
+        return (int)r.w[f];
+    else if constexpr (KB == 16)
+        return (int)(r.w[f >> 1] << (16 - 16 * (f & 1))) >> 16;
+    else if constexpr (KB == 8)
+        return (int)(r.w[f >> 2] << (24 - 8 * (f & 3))) >> 24;
+    else
+        return 0;
+}
+template <int KB>
+__device__ __forceinline__ void comp_rec_set(CompRec<KB>& r, int f, int raw)
+{
+    if constexpr (KB == 32) {
+        r.w[f] = (uint32_t)raw;
+    }
+    else if constexpr (KB == 16) {
+        const int s = 16 * (f & 1);
+        r.w[f >> 1] = (r.w[f >> 1] & ~(0xffffu << s)) | (((uint32_t)raw & 0xffffu) << s);
+    }
+    else if constexpr (KB == 8) {
+        const int s = 8 * (f & 3);
+        r.w[f >> 2] = (r.w[f >> 2] & ~(0xffu << s)) | (((uint32_t)raw & 0xffu) << s);
+    }
+}
+// Raw encoding of a compensation `comp` of the value f (the inverse of comp_decode).
+template <int KB>
+__device__ __forceinline__ int comp_encode(float comp, float f)
+{
+    if constexpr (KB == 32) {
+        return __float_as_int(comp);
     }
     else {
-        float q = rintf(ldexpf(comp, 8 - ulp_exp_f32(f)));
-        q = fminf(fmaxf(q, -127.f), 127.f);
-        *(int8_t*)p = (int8_t)q;
+        constexpr float lim = KB == 16 ? 32767.f : 127.f;
+        float q = rintf(ldexpf(comp, KB - ulp_exp_f32(f)));
+        q = fminf(fmaxf(q, -lim), lim);
+        return (int)q;
     }
 }
 template <int KB>
@@ -374,13 +402,15 @@ __device__ __forceinline__ float comp_decode(int raw, float f)
     return ldexpf((float)raw, ulp_exp_f32(f) - KB);
 }
 
-// As rk_update_plane, with w of the cell (wold; ignored at step 0) and, for KB > 0, the raw
-// compensation (craw) already loaded, so their latency hides behind the stencil work. boff is
-// the cell's byte offset in the plane of w and f (the compensation's follows from it).
+// The fused update addressed as (uniform plane base, 32-bit in-plane byte offset boff of the
+// cell in the plane of w and f), with w of the cell (wold; ignored at step 0) and, for KB > 0,
+// the raw compensation (craw) already loaded, so their latency hides behind the stencil work;
+// the new compensation's raw encoding is returned in *cenc (the caller stores the cell's
+// record).
 template <typename T, int KB>
-__device__ __forceinline__ T rk_update_pre(T* __restrict__ wp, void* __restrict__ cp,
-                                           T* __restrict__ op, unsigned boff, int step, T alpha,
-                                           T beta, T f, T rhs, T dt, T wold, int craw)
+__device__ __forceinline__ T rk_update_pre(T* __restrict__ wp, T* __restrict__ op, unsigned boff,
+                                           int step, T alpha, T beta, T f, T rhs, T dt, T wold,
+                                           int craw, int* cenc)
 {
     T wn = dt * rhs;
     if (step > 0)
@@ -396,7 +426,7 @@ __device__ __forceinline__ T rk_update_pre(T* __restrict__ wp, void* __restrict_
         const float y = beta * wn - cc;
         const float t = f + y;
         const float cn = (t - f) - y;
-        comp_store_b<KB>(cp, boff / (unsigned)sizeof(T) * (unsigned)(KB / 8), cn, t);
+        *cenc = comp_encode<KB>(cn, t);
         fn = t;
     }
     st_b(op, boff, fn);

@@ -142,11 +142,10 @@ __global__ void __launch_bounds__(TX* TY, (sizeof(T) == 4 ? 2 : 1))
 
         // w (and the Kahan compensation) of the output cell, loaded early; consumed at the end
         T wpre[NF];
-        int cpre[NF];
+        CompRec<KB> crec{};  // the output cell's compensation record (KB > 0)
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
             wpre[f] = T(0);
-            cpre[f] = 0;
         }
         if (out) {
             const long long poff = sz * (long long)(z + R);
@@ -155,13 +154,9 @@ __global__ void __launch_bounds__(TX* TY, (sizeof(T) == 4 ? 2 : 1))
                 for (int f = 0; f < NF; ++f)
                     wpre[f] = ld_b((const T*)(w0 + (f * fsb + poff * (long long)sizeof(T))), iob);
             }
-            if (KB > 0) {
-#pragma unroll
-                for (int f = 0; f < NF; ++f)
-                    cpre[f] = comp_raw_load<KB>(comp0 + (f * fsb + poff * (long long)sizeof(T)) /
-                                                            (long long)sizeof(T) * (KB / 8),
-                                                iob / (unsigned)sizeof(T) * (unsigned)(KB / 8));
-            }
+            if (KB > 0)
+                crec = comp_rec_load<KB>(comp0 + poff * (long long)(NF * KB / 8),
+                                         iob / (unsigned)sizeof(T) * (unsigned)(NF * KB / 8));
         }
 
         // ================= phase A: in-plane stencils of plane z (slot s0)
@@ -393,6 +388,7 @@ __global__ void __launch_bounds__(TX* TY, (sizeof(T) == 4 ? 2 : 1))
             mhd_rhs<T, UPWD, ENTROPY>(d, a.P, rhs);
             const long long poff = sz * (long long)(z + R);
             T fnew[NF];
+            CompRec<KB> cout = crec;  // ss keeps its record entry without entropy
 #pragma unroll
             for (int f = 0; f < NF; ++f) {
                 const long long fo = f * fsb + poff * (long long)sizeof(T);  // bytes
@@ -403,10 +399,15 @@ __global__ void __launch_bounds__(TX* TY, (sizeof(T) == 4 ? 2 : 1))
                     continue;
                 }
                 T* wp = (T*)(w0 + fo);
-                void* cp = KB ? (void*)(comp0 + fo / (long long)sizeof(T) * (KB / 8)) : nullptr;
-                fnew[f] = rk_update_pre<T, KB>(wp, cp, op, iob, a.step, a.alpha, a.beta, cen[f],
-                                               rhs[f], dt, wpre[f], cpre[f]);
+                int cenc = 0;
+                fnew[f] = rk_update_pre<T, KB>(wp, op, iob, a.step, a.alpha, a.beta, cen[f], rhs[f],
+                                               dt, wpre[f], comp_rec_get<KB>(crec, f), &cenc);
+                if (KB > 0)
+                    comp_rec_set<KB>(cout, f, cenc);
             }
+            if (KB > 0)
+                comp_rec_store<KB>(comp0 + poff * (long long)(NF * KB / 8),
+                                   iob / (unsigned)sizeof(T) * (unsigned)(NF * KB / 8), cout);
             if (a.step == 2) {
                 const double u2 = (double)fnew[1] * (double)fnew[1] + (double)fnew[2] * (double)fnew[2] +
                                   (double)fnew[3] * (double)fnew[3];

@@ -175,14 +175,13 @@ __device__ __forceinline__ void delay_line_body(const SolveArgs<T>& a, const Adj
         // w of this iteration's output cell, loaded early so its latency hides behind the
         // stencil work (it is needed only by the RK update at the end)
         T wpre[NF];
-        int cpre[NF];
+        CompRec<KB> crec{};  // the output cell's compensation record (KB > 0)
         T lfpre[NF];  // adjoint body: lambda_f and lambda_w of the output cell
         const int zo = p - R;
         const bool has_out = it >= 2 * R;
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
             wpre[f] = T(0);
-            cpre[f] = 0;
             lfpre[f] = T(0);
         }
         if (ADJ && has_out) {
@@ -201,13 +200,9 @@ __device__ __forceinline__ void delay_line_body(const SolveArgs<T>& a, const Adj
                 for (int f = 0; f < NF; ++f)
                     wpre[f] = ld_b((const T*)(w0 + (f * fsb + poff * (long long)sizeof(T))), iob);
             }
-            if (KB > 0) {
-#pragma unroll
-                for (int f = 0; f < NF; ++f)
-                    cpre[f] = comp_raw_load<KB>(comp0 + (f * fsb + poff * (long long)sizeof(T)) /
-                                                            (long long)sizeof(T) * (KB / 8),
-                                                iob / (unsigned)sizeof(T) * (unsigned)(KB / 8));
-            }
+            if (KB > 0)
+                crec = comp_rec_load<KB>(comp0 + poff * (long long)(NF * KB / 8),
+                                         iob / (unsigned)sizeof(T) * (unsigned)(NF * KB / 8));
         }
 
         Lds<T, LX> L[NF];
@@ -371,6 +366,7 @@ __device__ __forceinline__ void delay_line_body(const SolveArgs<T>& a, const Adj
             mhd_rhs<T, UPWD, ENTROPY>(d, a.P, rhs);
             const long long poff = sz * (long long)(z + R);
             T fnew[NF];
+            CompRec<KB> cout = crec;  // ss keeps its record entry without entropy
 #pragma unroll
             for (int f = 0; f < NF; ++f) {
                 const long long fo = f * fsb + poff * (long long)sizeof(T);  // bytes
@@ -381,10 +377,15 @@ __device__ __forceinline__ void delay_line_body(const SolveArgs<T>& a, const Adj
                     continue;
                 }
                 T* wp = (T*)(w0 + fo);
-                void* cp = KB ? (void*)(comp0 + fo / (long long)sizeof(T) * (KB / 8)) : nullptr;
-                fnew[f] = rk_update_pre<T, KB>(wp, cp, op, iob, a.step, a.alpha, a.beta, q[f][3],
-                                               rhs[f], dt, wpre[f], cpre[f]);
+                int cenc = 0;
+                fnew[f] = rk_update_pre<T, KB>(wp, op, iob, a.step, a.alpha, a.beta, q[f][3], rhs[f],
+                                               dt, wpre[f], comp_rec_get<KB>(crec, f), &cenc);
+                if (KB > 0)
+                    comp_rec_set<KB>(cout, f, cenc);
             }
+            if (KB > 0)
+                comp_rec_store<KB>(comp0 + poff * (long long)(NF * KB / 8),
+                                   iob / (unsigned)sizeof(T) * (unsigned)(NF * KB / 8), cout);
             if (a.step == 2) {
                 const double u2 = (double)fnew[1] * (double)fnew[1] + (double)fnew[2] * (double)fnew[2] +
                                   (double)fnew[3] * (double)fnew[3];
@@ -634,7 +635,7 @@ static bool uniform_slabs(const SolveArgs<T>& a, int kb)
         const long long o = q * a.fstride;
         if (a.in[q] != a.in[0] + o || a.out[q] != a.out[0] + o || a.w[q] != a.w[0] + o)
             return false;
-        if (kb > 0 && (const char*)a.comp[q] != (const char*)a.comp[0] + o * (kb / 8))
+        if (kb > 0 && (const char*)a.comp[q] != (const char*)a.comp[0] + q * (kb / 8))
             return false;
     }
     return true;

@@ -581,6 +581,8 @@ def main():
             try:
                 lib_v, dev_v = make_device(ac, prec, n, kb)
                 w_v, s_ms, cnt = timed(lib_v, dev_v, args.variant_steps, 3)
+                shp = (ctypes.c_int(), ctypes.c_int(), ctypes.c_int())
+                lib_v.acDeviceGetSolveLaunchShape(dev_v.handle, *[ctypes.byref(v) for v in shp])
                 dev_v.close()
             except Exception as e:  # report, do not hide
                 variants[f"{prec}_kahan{kb}"] = {"error": str(e)}
@@ -591,7 +593,8 @@ def main():
             variants[f"{prec}_kahan{kb}"] = {
                 "cell_updates_per_s": n3 * args.variant_steps / w_v,
                 "bytes_per_cell_update": algorithmic_bytes_per_cell_update(rb, kb),
-                "solve_achieved_GBs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4)}
+                "solve_achieved_GBs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4),
+                "launch_shape": [v.value for v in shp]}
         out["variants"] = variants
         fp64 = variants.get("fp64_kahan0", {}).get("cell_updates_per_s")
         if fp64:
