@@ -523,7 +523,10 @@ __global__ void __launch_bounds__(256) adjoint_transpose_lds_kernel(AdjArgs<T> A
     const int chunk = rest / nty;
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int tid = tx + TX * ty;
-    const int wave = tid >> 6, lane = tid & 63;
+    // wave-uniform: a scalar register in FP32 (scalar M0 arithmetic for the DMA); FP64 keeps
+    // the vector form, as the forward kernels do
+    const int wave = sizeof(T) == 4 ? __builtin_amdgcn_readfirstlane(tid >> 6) : tid >> 6;
+    const int lane = tid & 63;
     const int x0 = tile_x * TX, y0 = tile_y * TY, z0 = chunk * zchunk;
     const int sy = a.mx;
     const long long sz = a.sz;

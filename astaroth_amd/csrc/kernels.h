@@ -79,6 +79,13 @@ __device__ __forceinline__ long long opaque_s64(long long v)
     asm volatile("" : "+s"(v));
     return v;
 }
+// A per-field array base made to live in a scalar register pair: a global access then takes
+// the form uniform base + 32-bit lane offset instead of a per-lane 64-bit address built with
+// vector adds, which the compiler otherwise derives field by field from the previous one.
+__device__ __forceinline__ gchar* sbase(const void* p)
+{
+    return (gchar*)opaque_s64((long long)p);
+}
 // Candidate launch shapes of the tiled kernels (for the autotuner).
 int tiled_num_shapes();
 TiledShape tiled_shape(int i);

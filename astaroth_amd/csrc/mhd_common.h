@@ -310,6 +310,18 @@ __device__ __forceinline__ void st_b(T* base, unsigned boff, T v)
 {
     *(T*)((char*)base + boff) = v;
 }
+// The same through a base pointer known to be in global memory (see sbase).
+typedef __attribute__((address_space(1))) char gchar;
+template <typename T>
+__device__ __forceinline__ T ld_g(const gchar* base, unsigned boff)
+{
+    return *(const __attribute__((address_space(1))) T*)(base + boff);
+}
+template <typename T>
+__device__ __forceinline__ void st_b(gchar* base, unsigned boff, T v)
+{
+    *(__attribute__((address_space(1))) T*)(base + boff) = v;
+}
 
 // A cell's compensation record in registers: NF * KB / 32 dwords (2, 4 or 8).
 template <int KB>
@@ -407,8 +419,8 @@ __device__ __forceinline__ float comp_decode(int raw, float f)
 // the raw compensation (craw) already loaded, so their latency hides behind the stencil work;
 // the new compensation's raw encoding is returned in *cenc (the caller stores the cell's
 // record).
-template <typename T, int KB>
-__device__ __forceinline__ T rk_update_pre(T* __restrict__ wp, T* __restrict__ op, unsigned boff,
+template <typename T, int KB, typename P>
+__device__ __forceinline__ T rk_update_pre(P __restrict__ wp, P __restrict__ op, unsigned boff,
                                            int step, T alpha, T beta, T f, T rhs, T dt, T wold,
                                            int craw, int* cenc)
 {

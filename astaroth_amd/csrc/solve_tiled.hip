@@ -91,7 +91,14 @@ __device__ __forceinline__ void delay_line_body(const SolveArgs<T>& a, const Adj
 
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int tid = tx + TX * ty;
-    const int wave = tid >> 6, lane = tid & 63;
+    // The wave index is uniform across the wave. FP32: kept in a scalar register, so the LDS
+    // destinations of the DMA (M0) are scalar arithmetic (about 70 fewer vector instructions per
+    // plane step). FP64 keeps the per-DMA vector form: its kernels run at 512 VGPRs with spills,
+    // and the scalar form gave wrong results in the 64 x 2 periodic-image instantiation, with no
+    // fault found in the source (the code generation of these FP64 kernels is fragile; see the
+    // WRAP template parameter).
+    const int wave = sizeof(T) == 4 ? __builtin_amdgcn_readfirstlane(tid >> 6) : tid >> 6;
+    const int lane = tid & 63;
     const int x0 = tile_x * TX, y0 = tile_y * TY, z0 = chunk * zchunk;
     const long long sy = a.mx, sz = a.sz;
 
@@ -120,6 +127,22 @@ __device__ __forceinline__ void delay_line_body(const SolveArgs<T>& a, const Adj
     char* const out0 = (char*)a.out[0];
     char* const w0 = (char*)a.w[0];
     char* const comp0 = (char*)a.comp[0];
+    // Per-field bases of the RK arrays. With 16- and 32-bit compensation they are pinned to
+    // scalar registers (sbase), which keeps those kernels free of vector spills (about 3% and
+    // 20% faster at 256^3); the plain and 8-bit kernels are faster with the compiler's own choice.
+    constexpr bool SBASE = KB >= 16;
+    auto field_base = [](char* p) {
+        if constexpr (SBASE)
+            return sbase(p);
+        else
+            return (T*)p;
+    };
+    auto ld_field = [](char* p, unsigned boff) -> T {
+        if constexpr (SBASE)
+            return ld_g<T>(sbase(p), boff);
+        else
+            return ld_b((const T*)p, boff);
+    };
     // Issue the DMA of interior plane p (array z index p + R) into buffer b.
     auto dma_plane = [&](int p, int b, long long fsb) {
         if (WRAP)
@@ -198,7 +221,7 @@ __device__ __forceinline__ void delay_line_body(const SolveArgs<T>& a, const Adj
             if (a.step > 0) {
 #pragma unroll
                 for (int f = 0; f < NF; ++f)
-                    wpre[f] = ld_b((const T*)(w0 + (f * fsb + poff * (long long)sizeof(T))), iob);
+                    wpre[f] = ld_field(w0 + (f * fsb + poff * (long long)sizeof(T)), iob);
             }
             if (KB > 0)
                 crec = comp_rec_load<KB>(comp0 + poff * (long long)(NF * KB / 8),
@@ -370,13 +393,13 @@ __device__ __forceinline__ void delay_line_body(const SolveArgs<T>& a, const Adj
 #pragma unroll
             for (int f = 0; f < NF; ++f) {
                 const long long fo = f * fsb + poff * (long long)sizeof(T);  // bytes
-                T* op = (T*)(out0 + fo);
+                auto op = field_base(out0 + fo);
                 if (f == 7 && !ENTROPY) {
                     fnew[f] = q[f][3];
                     st_b(op, iob, q[f][3]);
                     continue;
                 }
-                T* wp = (T*)(w0 + fo);
+                auto wp = field_base(w0 + fo);
                 int cenc = 0;
                 fnew[f] = rk_update_pre<T, KB>(wp, op, iob, a.step, a.alpha, a.beta, q[f][3], rhs[f],
                                                dt, wpre[f], comp_rec_get<KB>(crec, f), &cenc);
