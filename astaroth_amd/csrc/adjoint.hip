@@ -796,12 +796,6 @@ void launch_adjoint(const AdjArgs<T>& A, bool upwd, bool entropy, double* param_
     // (C) transposed stencils, z-marching over chunks of planes: the LDS-staged form where the
     // grid fits its 64 x 4 column blocks (it reads the periodic images itself), else the plain
     // form after (B) a periodic fill of the derivative adjoints
-    int zc = A.s.nz;
-    for (int cand : {32, 16, 8})
-        if (A.s.nz % cand == 0 && A.s.nz > cand) {
-            zc = cand;
-            break;
-        }
     const bool staged = A.s.nx % 64 == 0 && A.s.ny % 4 == 0 && A.s.nz >= R;
     for (int k0 = 0; !staged && k0 < NG; k0 += NF) {
         T* ptrs[NF];
@@ -810,14 +804,22 @@ void launch_adjoint(const AdjArgs<T>& A, bool upwd, bool entropy, double* param_
             ptrs[q] = A.g + (long long)(k0 + q) * A.count;
         launch_periodic_fill<T>(ptrs, cnt, A.s.nx, A.s.ny, A.s.nz, s);
     }
+    // the plain form keeps the 32-plane chunks
+    int zc = A.s.nz;
+    for (int cand : {32, 16, 8})
+        if (A.s.nz % cand == 0 && A.s.nz > cand) {
+            zc = cand;
+            break;
+        }
     const dim3 bpz(bpg.x, bpg.y, (unsigned)(A.s.nz / zc));
+    const int zcs = staged ? adjoint_zchunk(A.s.nx, A.s.ny, A.s.nz) : zc;
     const int ntx = A.s.nx / 64, nty = A.s.ny / 4;
-    const dim3 bps((unsigned)(ntx * nty * (A.s.nz / zc)), 1, 1);
+    const dim3 bps((unsigned)(ntx * nty * (A.s.nz / zcs)), 1, 1);
 #define AC_ADJ_Q(Q)                                                                                \
     do {                                                                                           \
         if (staged) {                                                                              \
-            if (upwd) hipLaunchKernelGGL((adjoint_transpose_lds_kernel<T, true, Q>), bps, tpb, 0, s, A, zc, ntx, nty); \
-            else hipLaunchKernelGGL((adjoint_transpose_lds_kernel<T, false, Q>), bps, tpb, 0, s, A, zc, ntx, nty);     \
+            if (upwd) hipLaunchKernelGGL((adjoint_transpose_lds_kernel<T, true, Q>), bps, tpb, 0, s, A, zcs, ntx, nty); \
+            else hipLaunchKernelGGL((adjoint_transpose_lds_kernel<T, false, Q>), bps, tpb, 0, s, A, zcs, ntx, nty);    \
         }                                                                                          \
         else if (upwd) hipLaunchKernelGGL((adjoint_transpose_z_kernel<T, true, Q>), bpz, tpb, 0, s, A, zc); \
         else hipLaunchKernelGGL((adjoint_transpose_z_kernel<T, false, Q>), bpz, tpb, 0, s, A, zc);    \

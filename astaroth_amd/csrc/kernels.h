@@ -86,6 +86,20 @@ __device__ __forceinline__ gchar* sbase(const void* p)
 {
     return (gchar*)opaque_s64((long long)p);
 }
+// z-chunk of the adjoint's tiled kernels (64 x 4 column blocks): long chunks amortise the six
+// plane steps of each chunk's start-up; the longest of 128 and 64 that still gives at least 512
+// workgroups, else the former choice, the largest of 32, 16 and 8 below nz.
+inline int adjoint_zchunk(int nx, int ny, int nz)
+{
+    const long long tiles = (long long)(nx / 64) * (ny / 4);
+    for (int cand : {128, 64})
+        if (nz % cand == 0 && nz > cand && tiles * (nz / cand) >= 512)
+            return cand;
+    for (int cand : {32, 16, 8})
+        if (nz % cand == 0 && nz > cand)
+            return cand;
+    return nz;
+}
 // Candidate launch shapes of the tiled kernels (for the autotuner).
 int tiled_num_shapes();
 TiledShape tiled_shape(int i);

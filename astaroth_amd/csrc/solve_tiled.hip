@@ -705,12 +705,7 @@ bool launch_adjoint_local_tiled(const SolveArgs<T>& a, const AdjLocal<T>& x, boo
 {
     if (a.nx % 64 != 0 || a.ny % 4 != 0 || !uniform_slabs(a, 0))
         return false;
-    int zc = a.nz;
-    for (int cand : {32, 16, 8})
-        if (a.nz % cand == 0 && a.nz > cand) {
-            zc = cand;
-            break;
-        }
+    const int zc = adjoint_zchunk(a.nx, a.ny, a.nz);
     const int ntx = a.nx / 64, nty = a.ny / 4, nch = a.nz / zc;
     const dim3 tpb(64, 4, 1), bpg((unsigned)(ntx * nty * nch), 1, 1);
     *nblocks = (long long)ntx * nty * nch;
