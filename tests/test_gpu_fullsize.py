@@ -143,3 +143,19 @@ def test_dot_product_128(prec, tol, require_gpu):
         lhs, rhs, rel, pg = TA.substep_dot_product(lib, info, f, step, True, 17 + step)
         say(f"case dot-product 128^3 {prec} substep {step}: rel = {rel:.3g}")
         assert rel < tol
+
+
+@pytest.mark.parametrize("shape,chunk", [((256, 256, 256), 128), ((256, 256, 128), 64)])
+def test_dot_product_long_chunks(shape, chunk, require_gpu):
+    """Grids whose 64 x 4 column blocks give 512 workgroups with long z-chunks: the adjoint's
+    local stage and transposes march 128-plane (256^3) and 64-plane (256^2 x 128) chunks there,
+    and the substep adjoint passes the dot-product test at 1e-6 in FP32."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import test_gpu_adjoint as TA
+    lib = ac.load("fp32")
+    info = TA.physics_info(lib, shape)
+    f = TA.initial(lib, info, 7)
+    for step in (1,):
+        lhs, rhs, rel, pg = TA.substep_dot_product(lib, info, f, step, True, 29 + step)
+        say(f"case dot-product {shape} ({chunk}-plane chunks) fp32 substep {step}: rel = {rel:.3g}")
+        assert rel < 1e-6
