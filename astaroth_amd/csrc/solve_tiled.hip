@@ -72,6 +72,13 @@ __device__ __forceinline__ void delay_line_body(const SolveArgs<T>& a, const Adj
     // neighbours, whose rounding in FP32 is of order eps |f| / dx^2 against a result of order
     // k^2 |f|; about 10x smaller errors in u, A and lnrho for about 18% more VALU work
     constexpr bool CANCEL_FREE = sizeof(T) == 4 && !ADJ && KB == 32;
+    // LATE: the output of plane p - 3 (with its stores) runs after the end-of-step barrier, and
+    // the DMA runs two planes ahead, issued right after that barrier into the buffer just
+    // released. The barrier's wait then finds the previous step's stores and the next plane's
+    // DMA long done, instead of waiting for the stores just issued. Used by the adjoint's local
+    // stage (55 stores per cell, one workgroup per CU) in FP32; in FP64 the longer-lived
+    // registers would spill.
+    constexpr bool LATE = ADJ && sizeof(T) == 4;
     static_assert(NT % 64 == 0, "workgroup must be whole waves");
     static_assert((NR * NW * 64) % WPE == 0, "field-plane stride");
     __shared__ T lds[2][NF * LPS];
@@ -185,6 +192,8 @@ __device__ __forceinline__ void delay_line_body(const SolveArgs<T>& a, const Adj
     const unsigned ioff = (unsigned)((gx + R) + (int)sy * (gy + R));  // in-plane offset
 
     dma_plane(z0 - R, 0, a.fstride * (long long)sizeof(T));
+    if (LATE)
+        dma_plane(z0 - R + 1, 1, a.fstride * (long long)sizeof(T));
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     const int niter = zchunk + 2 * R;
@@ -193,7 +202,7 @@ __device__ __forceinline__ void delay_line_body(const SolveArgs<T>& a, const Adj
         const int cur = it & 1;
         const unsigned iob = opaque(ioff * (unsigned)sizeof(T));  // byte offset in a plane
         const long long fsb = opaque_s64(a.fstride * (long long)sizeof(T));  // field stride, bytes
-        if (it + 1 < niter)
+        if (!LATE && it + 1 < niter)
             dma_plane(p + 1, cur ^ 1, fsb);
         // w of this iteration's output cell, loaded early so its latency hides behind the
         // stencil work (it is needed only by the RK update at the end)
@@ -277,14 +286,16 @@ __device__ __forceinline__ void delay_line_body(const SolveArgs<T>& a, const Adj
             }
         }
 
-        // ---- (2) output plane z = p - 3
+        // ---- (2) output plane z = p - 3: here, or after the end-of-step barrier when LATE
         const int z = p - R;
-        if (it >= 2 * R) {
-            const PlaneVals<T>& o = dly[0];
-            T dv[NDL];  // the A / ss values of plane z, written three steps ago
+        T dv[NDL];  // the A / ss values of plane z, written three steps ago
+        auto read_dv = [&]() __attribute__((always_inline)) {
 #pragma unroll
             for (int k = 0; k < NDL; ++k)
                 dv[k] = dl[dslot][k][tid];
+        };
+        auto output_phase = [&]() __attribute__((always_inline)) {
+            const PlaneVals<T>& o = dly[0];
             T dz[NF], dzz[NF], upz[4];
 #pragma unroll
             for (int f = 0; f < NF; ++f) {
@@ -415,7 +426,17 @@ __device__ __forceinline__ void delay_line_body(const SolveArgs<T>& a, const Adj
                 u2max = fmax(u2max, u2);
             }
             }
+        };
+        if constexpr (!LATE) {
+            if (it >= 2 * R) {
+                read_dv();
+                output_phase();
+            }
         }
+        else if (it >= 2 * R) {
+            read_dv();  // before (3) overwrites the ring slot with plane p
+        }
+
 
         // ---- (3) remaining in-plane derivatives of plane p
         const T aux = ac_abs(q[1][6]), auy = ac_abs(q[2][6]);
@@ -524,6 +545,18 @@ __device__ __forceinline__ void delay_line_body(const SolveArgs<T>& a, const Adj
             ch[3 * v + 1][3] += mxx;  // d_xy v_x into (grad div v)_y
         }
 
+        if constexpr (LATE) {
+            // every wave is done with buffer cur (plane p) and plane p + 1 has landed in the
+            // other one; the DMA of plane p + 2 into cur then overlaps the output work, and the
+            // output's stores have the next step's stencil work to complete before its barrier
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            if (it + 2 < niter)
+                dma_plane(p + 2, cur, fsb);
+            if (it >= 2 * R)
+                output_phase();
+        }
+
         // ---- (4) roll the register state by one plane
 #pragma unroll
         for (int f = 0; f < NF; ++f)
@@ -544,9 +577,11 @@ __device__ __forceinline__ void delay_line_body(const SolveArgs<T>& a, const Adj
         dly[2] = dly[3];
         dslot = dslot == 2 ? 0 : dslot + 1;
 
-        // plane p + 1 has landed in the other buffer and every wave is done with this one
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
+        if constexpr (!LATE) {
+            // plane p + 1 has landed in the other buffer and every wave is done with this one
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+        }
     }
     if constexpr (ADJ) {
         // block sums of the parameter sensitivities (FP64, deterministic)

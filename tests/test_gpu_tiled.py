@@ -73,7 +73,7 @@ def skip_ring_fp64(lib, shape):
                                      ((32, 32, 64), (32, 8, 64)), ((32, 32, 32), (32, 8, 32)),
                                      ((32, 48, 64), (16, 16, 32)), ((32, 32, 32), (32, 16, 32)),
                                      ((64, 48, 64), (32, 16, 64))])
-@pytest.mark.parametrize("upwd,entropy", [(True, True), (False, False)])
+@pytest.mark.parametrize("upwd,entropy", [(True, True), (False, False), (True, False), (False, True)])
 def test_tiled_vs_oracle(lib, n, shape, upwd, entropy):
     skip_ring_fp64(lib, shape)
     say(f"case tiled vs oracle {lib.precision} n={n} shape={shape} upwd={upwd} entropy={entropy}")
@@ -139,12 +139,13 @@ def test_tiled_kahan_closer_to_fp64(kb, shape, require_gpu):
                                      ((128, 4, 8), (64, 4, 8)), ((32, 32, 32), (32, 8, 32)),
                                      ((64, 16, 32), (-1, 0, 0))])
 @pytest.mark.parametrize("mode", ["courant", "reference"])
-def test_fused_periodic_steps(lib, n, shape, mode):
+def test_fused_periodic_steps(lib, n, shape, mode, upwd=True, entropy=True):
     """acDeviceIntegrateStepsCFL runs the delay-line kernels on the periodic images of the halo
     cells (no fill before a substep); the ring and direct kernels get a fill first. Either way the
     result equals fill + substep + swap three times per step, bit for bit."""
-    say(f"case fused periodic steps {lib.precision} n={n} shape={shape} mode={mode}")
-    info = physics_info(lib, n, shape)
+    say(f"case fused periodic steps {lib.precision} n={n} shape={shape} mode={mode} "
+        f"upwd={upwd} entropy={entropy}")
+    info = physics_info(lib, n, shape, AC_lupwd=upwd, AC_lentropy=entropy)
     data = initial(lib, info, seed=11)
     courant = lib.np_real(0.3)
     tmode = ac.AC_TIMESTEP_COURANT if mode == "courant" else ac.AC_TIMESTEP_REFERENCE
@@ -176,3 +177,11 @@ def test_fused_periodic_steps(lib, n, shape, mode):
     assert len(diff) == 0, diff[:5]
     assert outs[0][1] == outs[1][1]
     assert np.isfinite(outs[0][0]).all()
+
+
+@pytest.mark.parametrize("n,shape", [((64, 16, 32), (64, 4, 16)), ((64, 8, 16), (64, 2, 8))])
+@pytest.mark.parametrize("upwd,entropy", [(False, False), (True, False), (False, True)])
+def test_fused_periodic_steps_switches(lib, n, shape, upwd, entropy):
+    """The periodic-image kernels of the other physics switches (each a separate instantiation)
+    against fill + substep + swap, bit for bit."""
+    test_fused_periodic_steps(lib, n, shape, "courant", upwd, entropy)
