@@ -431,10 +431,11 @@ __global__ void __launch_bounds__(256) adjoint_transpose_z_kernel(AdjArgs<T> A, 
 }
 
 // ---------------------------------------------------------------- adjoint (C), LDS-staged form
-// The same per-output-field transposes on 64 x 4 column blocks: the derivative-adjoint fields
-// output field Q reads with in-plane (x / y) neighbours are DMA'd plane by plane into LDS (with
-// their radius-3 halo, double-buffered, as the forward solve stages its planes), fields read
-// only at the column come straight from memory, and the z parts use the same push chain.
+// The same per-output-field transposes on 64 x 8 (or 64 x 4) column blocks: the
+// derivative-adjoint fields output field Q reads with in-plane (x / y) neighbours are DMA'd
+// plane by plane into LDS (with their radius-3 halo, double-buffered, as the forward solve
+// stages its planes), fields read only at the column come straight from memory, and the z parts
+// use the same push chain.
 
 // Fields of output Q that are read with in-plane neighbours ("staged"), in slot order.
 __host__ __device__ constexpr int staged_count(int Q, bool upwd)
@@ -493,11 +494,13 @@ __host__ __device__ constexpr int column_field(int Q, bool upwd, int slot)
     return slot == 0 ? G_S : G_S + 3;
 }
 
-template <typename T, bool UPWD, int Q>
-__global__ void __launch_bounds__(256) adjoint_transpose_lds_kernel(AdjArgs<T> A, int zchunk,
-                                                                    int ntx, int nty)
+// TY = 8 where ny allows (512-thread workgroups: the in-plane halo re-read from L2 drops from
+// 10/4 to 14/8 rows per row and FP64 runs two waves per SIMD instead of one), else 4.
+template <typename T, bool UPWD, int Q, int TY>
+__global__ void __launch_bounds__(64 * TY) adjoint_transpose_lds_kernel(AdjArgs<T> A, int zchunk,
+                                                                        int ntx, int nty)
 {
-    constexpr int TX = 64, TY = 4;
+    constexpr int TX = 64;
     constexpr int LX = TX + 2 * R, LY = TY + 2 * R;
     constexpr int LP = LX * LY;
     constexpr int NT = TX * TY, NW = NT / 64;
@@ -804,7 +807,7 @@ void launch_adjoint(const AdjArgs<T>& A, bool upwd, bool entropy, double* param_
             ptrs[q] = A.g + (long long)(k0 + q) * A.count;
         launch_periodic_fill<T>(ptrs, cnt, A.s.nx, A.s.ny, A.s.nz, s);
     }
-    // the plain form keeps the 32-plane chunks
+    // 32-plane chunks (the transposes' time hardly depends on the chunk length)
     int zc = A.s.nz;
     for (int cand : {32, 16, 8})
         if (A.s.nz % cand == 0 && A.s.nz > cand) {
@@ -812,14 +815,18 @@ void launch_adjoint(const AdjArgs<T>& A, bool upwd, bool entropy, double* param_
             break;
         }
     const dim3 bpz(bpg.x, bpg.y, (unsigned)(A.s.nz / zc));
-    const int zcs = staged ? adjoint_zchunk(A.s.nx, A.s.ny, A.s.nz) : zc;
-    const int ntx = A.s.nx / 64, nty = A.s.ny / 4;
-    const dim3 bps((unsigned)(ntx * nty * (A.s.nz / zcs)), 1, 1);
+    const bool ty8 = A.s.ny % 8 == 0;
+    const int ntx = A.s.nx / 64, nty = A.s.ny / (ty8 ? 8 : 4);
+    const dim3 bps((unsigned)(ntx * nty * (A.s.nz / zc)), 1, 1), tpb8(64, 8, 1);
 #define AC_ADJ_Q(Q)                                                                                \
     do {                                                                                           \
-        if (staged) {                                                                              \
-            if (upwd) hipLaunchKernelGGL((adjoint_transpose_lds_kernel<T, true, Q>), bps, tpb, 0, s, A, zcs, ntx, nty); \
-            else hipLaunchKernelGGL((adjoint_transpose_lds_kernel<T, false, Q>), bps, tpb, 0, s, A, zcs, ntx, nty);    \
+        if (staged && ty8) {                                                                       \
+            if (upwd) hipLaunchKernelGGL((adjoint_transpose_lds_kernel<T, true, Q, 8>), bps, tpb8, 0, s, A, zc, ntx, nty); \
+            else hipLaunchKernelGGL((adjoint_transpose_lds_kernel<T, false, Q, 8>), bps, tpb8, 0, s, A, zc, ntx, nty);    \
+        }                                                                                          \
+        else if (staged) {                                                                         \
+            if (upwd) hipLaunchKernelGGL((adjoint_transpose_lds_kernel<T, true, Q, 4>), bps, tpb, 0, s, A, zc, ntx, nty); \
+            else hipLaunchKernelGGL((adjoint_transpose_lds_kernel<T, false, Q, 4>), bps, tpb, 0, s, A, zc, ntx, nty);    \
         }                                                                                          \
         else if (upwd) hipLaunchKernelGGL((adjoint_transpose_z_kernel<T, true, Q>), bpz, tpb, 0, s, A, zc); \
         else hipLaunchKernelGGL((adjoint_transpose_z_kernel<T, false, Q>), bpz, tpb, 0, s, A, zc);    \

@@ -86,7 +86,7 @@ __device__ __forceinline__ gchar* sbase(const void* p)
 {
     return (gchar*)opaque_s64((long long)p);
 }
-// z-chunk of the adjoint's tiled kernels (64 x 4 column blocks): long chunks amortise the six
+// z-chunk of the adjoint's local stage (64 x 4 column blocks): long chunks amortise the six
 // plane steps of each chunk's start-up; the longest of 128 and 64 that still gives at least 512
 // workgroups, else the former choice, the largest of 32, 16 and 8 below nz.
 inline int adjoint_zchunk(int nx, int ny, int nz)
