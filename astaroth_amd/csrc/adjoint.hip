@@ -431,7 +431,7 @@ __global__ void __launch_bounds__(256) adjoint_transpose_z_kernel(AdjArgs<T> A, 
 }
 
 // ---------------------------------------------------------------- adjoint (C), LDS-staged form
-// The same per-output-field transposes on 64 x 8 (or 64 x 4) column blocks: the
+// The same per-output-field transposes on 64 x 16, 64 x 8 or 64 x 4 column blocks: the
 // derivative-adjoint fields output field Q reads with in-plane (x / y) neighbours are DMA'd
 // plane by plane into LDS (with their radius-3 halo, double-buffered, as the forward solve
 // stages its planes), fields read only at the column come straight from memory, and the z parts
@@ -494,8 +494,9 @@ __host__ __device__ constexpr int column_field(int Q, bool upwd, int slot)
     return slot == 0 ? G_S : G_S + 3;
 }
 
-// TY = 8 where ny allows (512-thread workgroups: the in-plane halo re-read from L2 drops from
-// 10/4 to 14/8 rows per row and FP64 runs two waves per SIMD instead of one), else 4.
+// TY = 16 (FP32) or 8 where ny allows (1024- or 512-thread workgroups: the staged in-plane halo
+// drops from 10/4 to 22/16 or 14/8 rows per output row, and FP64 runs two waves per SIMD
+// instead of one), else 4.
 template <typename T, bool UPWD, int Q, int TY>
 __global__ void __launch_bounds__(64 * TY) adjoint_transpose_lds_kernel(AdjArgs<T> A, int zchunk,
                                                                         int ntx, int nty)
@@ -815,12 +816,18 @@ void launch_adjoint(const AdjArgs<T>& A, bool upwd, bool entropy, double* param_
             break;
         }
     const dim3 bpz(bpg.x, bpg.y, (unsigned)(A.s.nz / zc));
-    const bool ty8 = A.s.ny % 8 == 0;
-    const int ntx = A.s.nx / 64, nty = A.s.ny / (ty8 ? 8 : 4);
-    const dim3 bps((unsigned)(ntx * nty * (A.s.nz / zc)), 1, 1), tpb8(64, 8, 1);
+    // the transposes' column-block height: 16 (FP32) or 8 where ny allows, else 4
+    constexpr int TYB = sizeof(T) == 4 ? 16 : 8;
+    const int tyc = A.s.ny % TYB == 0 ? TYB : (A.s.ny % 8 == 0 ? 8 : 4);
+    const int ntx = A.s.nx / 64, nty = A.s.ny / tyc;
+    const dim3 bps((unsigned)(ntx * nty * (A.s.nz / zc)), 1, 1), tpbb(64, TYB, 1), tpb8(64, 8, 1);
 #define AC_ADJ_Q(Q)                                                                                \
     do {                                                                                           \
-        if (staged && ty8) {                                                                       \
+        if (staged && tyc == TYB) {                                                                \
+            if (upwd) hipLaunchKernelGGL((adjoint_transpose_lds_kernel<T, true, Q, TYB>), bps, tpbb, 0, s, A, zc, ntx, nty); \
+            else hipLaunchKernelGGL((adjoint_transpose_lds_kernel<T, false, Q, TYB>), bps, tpbb, 0, s, A, zc, ntx, nty);    \
+        }                                                                                          \
+        else if (staged && tyc == 8) {                                                             \
             if (upwd) hipLaunchKernelGGL((adjoint_transpose_lds_kernel<T, true, Q, 8>), bps, tpb8, 0, s, A, zc, ntx, nty); \
             else hipLaunchKernelGGL((adjoint_transpose_lds_kernel<T, false, Q, 8>), bps, tpb8, 0, s, A, zc, ntx, nty);    \
         }                                                                                          \

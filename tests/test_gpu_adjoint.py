@@ -109,7 +109,9 @@ def substep_dot_product(lib, info, f, step, entropy, seed):
 
 @pytest.mark.parametrize("step", [0, 1, 2])
 @pytest.mark.parametrize("upwd,entropy", [(True, True), (False, False), (True, False)])
-@pytest.mark.parametrize("n", [16, (64, 8, 16)])  # one-cell-per-thread and tiled local stage
+# one-cell-per-thread and tiled local stage; the tiled grids' ny select the transposes' column
+# block heights: 8, 4 and (FP32) 16
+@pytest.mark.parametrize("n", [16, (64, 8, 16), (64, 12, 16), (64, 16, 16)])
 def test_substep_dot_product(lib, step, upwd, entropy, n):
     say(f"case dot-product substep {step} {lib.precision} upwd={upwd} entropy={entropy} n={n}")
     info = physics_info(lib, n, AC_lupwd=upwd, AC_lentropy=entropy)
