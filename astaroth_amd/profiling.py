@@ -9,7 +9,8 @@ substep); `group` and `traffic` then combine them, weighted by their dispatch co
 
 `traffic` reads FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of the matching kernels, scales each
 by what the same counter reads for the device-to-device copy of `copy_bytes` bytes recorded in
-the same run (a kernel whose true traffic is copy_bytes read + copy_bytes written), and prints a
+the same run (a kernel whose true traffic is copy_bytes read + copy_bytes written; the largest
+copy dispatch of the run), and prints a
 JSON record with HBM bytes per launch.
 """
 import json
@@ -35,6 +36,17 @@ def counter_means(db, counter):
     return {name: (n, val) for name, n, val in rows}
 
 
+def counter_max(db, counter, sub):
+    """The largest per-dispatch value of a counter over the kernels whose name contains sub."""
+    con = sqlite3.connect(db)
+    rows = con.execute("select name, counter_value from pmc_events where counter_name = ?",
+                       (counter,))
+    vals = [v for name, v in rows if sub in name]
+    if not vals:
+        raise KeyError(f"no kernel matching {sub!r}")
+    return max(vals)
+
+
 def traffic(fetch_db, write_db, copy_bytes, kernel_sub):
     f = counter_means(fetch_db, "FETCH_SIZE")
     w = counter_means(write_db, "WRITE_SIZE")
@@ -47,8 +59,9 @@ def traffic(fetch_db, write_db, copy_bytes, kernel_sub):
         n = sum(d[k][0] for k in keys)
         return keys, n, sum(d[k][0] * d[k][1] for k in keys) / n
 
-    _, _, copy_f = pick(f, "copyBuffer")
-    _, _, copy_w = pick(w, "copyBuffer")
+    # the calibration copy is the largest copy of the run (a run may hold many small ones)
+    copy_f = counter_max(fetch_db, "FETCH_SIZE", "copyBuffer")
+    copy_w = counter_max(write_db, "WRITE_SIZE", "copyBuffer")
     kf, nf, vf = pick(f, kernel_sub)
     _, _, vw = pick(w, kernel_sub)
     scale_f = copy_bytes / (copy_f * 1024.0)

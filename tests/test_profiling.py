@@ -34,10 +34,12 @@ def test_traffic_weights_by_dispatches_and_scales_by_copy(tmp_path):
     kib = 1024.0
     fetch = make_db(tmp_path / "f.db", pmc=[
         ("__amd_rocclr_copyBuffer", "FETCH_SIZE", 512.0),   # half of the 1 MiB copy counted
+        ("__amd_rocclr_copyBuffer", "FETCH_SIZE", 3.0),     # a small copy of the same run
         ("solve<0>", "FETCH_SIZE", 100.0), ("solve<0>", "FETCH_SIZE", 100.0),
         ("solve<2>", "FETCH_SIZE", 400.0)])
     write = make_db(tmp_path / "w.db", pmc=[
         ("__amd_rocclr_copyBuffer", "WRITE_SIZE", 1024.0),
+        ("__amd_rocclr_copyBuffer", "WRITE_SIZE", 6.0),
         ("solve<0>", "WRITE_SIZE", 300.0), ("solve<0>", "WRITE_SIZE", 300.0),
         ("solve<2>", "WRITE_SIZE", 0.0)])
     r = P.traffic(fetch, write, int(1024 * kib), "solve<")
