@@ -139,13 +139,13 @@ def test_tiled_kahan_closer_to_fp64(kb, shape, require_gpu):
                                      ((128, 4, 8), (64, 4, 8)), ((32, 32, 32), (32, 8, 32)),
                                      ((64, 16, 32), (-1, 0, 0))])
 @pytest.mark.parametrize("mode", ["courant", "reference"])
-def test_fused_periodic_steps(lib, n, shape, mode, upwd=True, entropy=True):
+def test_fused_periodic_steps(lib, n, shape, mode, upwd=True, entropy=True, kb=0):
     """acDeviceIntegrateStepsCFL runs the delay-line kernels on the periodic images of the halo
     cells (no fill before a substep); the ring and direct kernels get a fill first. Either way the
     result equals fill + substep + swap three times per step, bit for bit."""
     say(f"case fused periodic steps {lib.precision} n={n} shape={shape} mode={mode} "
-        f"upwd={upwd} entropy={entropy}")
-    info = physics_info(lib, n, shape, AC_lupwd=upwd, AC_lentropy=entropy)
+        f"upwd={upwd} entropy={entropy} kahan={kb}")
+    info = physics_info(lib, n, shape, AC_lupwd=upwd, AC_lentropy=entropy, AC_kahan_bits=kb)
     data = initial(lib, info, seed=11)
     courant = lib.np_real(0.3)
     tmode = ac.AC_TIMESTEP_COURANT if mode == "courant" else ac.AC_TIMESTEP_REFERENCE
@@ -185,3 +185,13 @@ def test_fused_periodic_steps_switches(lib, n, shape, upwd, entropy):
     """The periodic-image kernels of the other physics switches (each a separate instantiation)
     against fill + substep + swap, bit for bit."""
     test_fused_periodic_steps(lib, n, shape, "courant", upwd, entropy)
+
+
+@pytest.mark.parametrize("n,shape", [((64, 16, 32), (64, 4, 16)), ((64, 8, 16), (64, 2, 8))])
+@pytest.mark.parametrize("kb", [8, 16, 32])
+def test_fused_periodic_steps_kahan(n, shape, kb, require_gpu):
+    """The compensated FP32 kernels (one instantiation per substep and compensation width) on
+    the periodic images against fill + substep + swap, bit for bit, in both timestep modes."""
+    lib = ac.load("fp32")
+    for mode in ("courant", "reference"):
+        test_fused_periodic_steps(lib, n, shape, mode, kb=kb)
