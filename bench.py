@@ -96,6 +96,22 @@ def timed(lib, dev, steps, warmup, courant=0.2):
     return t1 - t0, tot.value, cnt.value
 
 
+def reference_call_pattern(ac, n, steps=20, warmup=3, dt=1e-3):
+    """The drop-in loop a caller of the reference API writes (ref samples/devicetest/main.cc:
+    98-103): acDevicePeriodicBoundconds + acDeviceIntegrateSubstep + acDeviceSwapBuffers per
+    substep, fixed dt, FP32, no fused multi-step path. Host-timed around device syncs."""
+    lib, dev = make_device(ac, "fp32", n)
+    try:
+        dt = lib.np_real(dt)
+        fixed_steps(dev, dt, warmup)
+        wall = fixed_steps(dev, dt, steps)
+    finally:
+        dev.close()
+    return {"cell_updates_per_s": n ** 3 * steps / wall, "steps": steps, "dt": float(dt),
+            "calls": "acDevicePeriodicBoundconds + acDeviceIntegrateSubstep + "
+                     "acDeviceSwapBuffers per substep"}
+
+
 def cpu_baseline(n=96, min_seconds=10.0, max_steps=1000):
     """ref-cpu: the reference's CPU-build algorithm (two-pass RK3, FP64), restated in C++ with
     OpenMP over z on every host core. Bounded sample: n^3 cells, steps until min_seconds."""
@@ -596,6 +612,10 @@ def main():
                 "solve_achieved_GBs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4),
                 "launch_shape": [v.value for v in shp]}
         out["variants"] = variants
+        try:
+            out["reference_call_pattern"] = reference_call_pattern(ac, n)
+        except Exception as e:  # report, do not hide
+            out["reference_call_pattern"] = {"error": str(e)}
         fp64 = variants.get("fp64_kahan0", {}).get("cell_updates_per_s")
         if fp64:
             out["fp32_over_fp64"] = value / fp64
