@@ -8,6 +8,9 @@
 // Reductions: ref src/core/kernels/reductions.cu:98-219 (map kernel into an n^3 scratchpad, then
 // hipcub DeviceReduce). Here the map is fused into a single pass with FP64 accumulation and a
 // deterministic two-stage tree (fixed grid, fixed order).
+#include <cstdio>
+#include <cstdlib>
+
 #include "kernels.h"
 
 namespace acamd {
@@ -20,63 +23,65 @@ template <typename T>
 struct FillArgs {
     T* f[NF];
     int nx, ny, nz, mx, my;
-    long long halo_count;  // per field
+    unsigned halo_count;  // per field (< 2^32: checked at launch)
+    int wide;             // every extent >= R: a halo cell's image is one period away
 };
 
+// One thread per halo cell and field. The cell enumeration (two z slabs of 3 full planes, then
+// per interior plane 3 + 3 full rows and 3 + 3 cells at the ends of each interior row) is
+// inverted with 32-bit divisions, and the image is the cell one period away (wrapped with a
+// true modulo only on grids narrower than the halo), so the kernel is bound by its memory
+// traffic rather than by 64-bit integer division.
 template <typename T>
 __global__ void __launch_bounds__(256) periodic_fill_kernel(FillArgs<T> a)
 {
-    const long long t0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned t0 = blockIdx.x * blockDim.x + threadIdx.x;
     if (t0 >= a.halo_count)
         return;
     T* __restrict__ f = a.f[blockIdx.y];
-    const long long plane = (long long)a.mx * a.my;
+    const unsigned mx = (unsigned)a.mx, plane = mx * (unsigned)a.my;
     int i, j, k;
-    long long t = t0;
-    if (t < 3 * plane) {
-        k = (int)(t / plane);
-        const long long r = t - (long long)k * plane;
-        j = (int)(r / a.mx);
-        i = (int)(r - (long long)j * a.mx);
-    }
-    else if (t < 6 * plane) {
-        t -= 3 * plane;
-        const int kk = (int)(t / plane);
-        const long long r = t - (long long)kk * plane;
-        k = a.nz + R + kk;
-        j = (int)(r / a.mx);
-        i = (int)(r - (long long)j * a.mx);
+    unsigned t = t0;
+    if (t < 6u * plane) {
+        const unsigned kk = t / plane;
+        const unsigned r = t - kk * plane;
+        k = kk < 3u ? (int)kk : a.nz + (int)kk;  // planes 0..2, then nz + 3 .. nz + 5
+        j = (int)(r / mx);
+        i = (int)(r - (unsigned)j * mx);
     }
     else {
-        t -= 6 * plane;
-        const long long per = 6LL * a.mx + 6LL * a.ny;
-        const int kk = (int)(t / per);
-        int r = (int)(t - (long long)kk * per);
-        k = R + kk;
-        if (r < 3 * a.mx) {
-            j = r / a.mx;
-            i = r - j * a.mx;
-        }
-        else if (r < 6 * a.mx) {
-            r -= 3 * a.mx;
-            const int jj = r / a.mx;
-            j = a.ny + R + jj;
-            i = r - jj * a.mx;
+        t -= 6u * plane;
+        const unsigned per = 6u * mx + 6u * (unsigned)a.ny;
+        const unsigned kk = t / per;
+        unsigned r = t - kk * per;
+        k = R + (int)kk;
+        if (r < 6u * mx) {
+            const unsigned jj = r / mx;
+            j = jj < 3u ? (int)jj : a.ny + (int)jj;  // rows 0..2, then ny + 3 .. ny + 5
+            i = (int)(r - jj * mx);
         }
         else {
-            r -= 6 * a.mx;
-            const int row = r / 6;
-            const int cc = r - row * 6;
-            j = R + row;
+            r -= 6u * mx;
+            const unsigned row = r / 6u;
+            const int cc = (int)(r - row * 6u);
+            j = R + (int)row;
             i = cc < 3 ? cc : a.nx + cc;
         }
     }
-    // proper modulo: also correct when an extent is smaller than the halo width
-    const int is = ((i - R) % a.nx + a.nx) % a.nx + R;
-    const int js = ((j - R) % a.ny + a.ny) % a.ny + R;
-    const int ks = ((k - R) % a.nz + a.nz) % a.nz + R;
-    const long long dst = (long long)i + (long long)a.mx * j + plane * k;
-    const long long src = (long long)is + (long long)a.mx * js + plane * ks;
+    int is, js, ks;
+    if (a.wide) {
+        is = i < R ? i + a.nx : (i >= a.nx + R ? i - a.nx : i);
+        js = j < R ? j + a.ny : (j >= a.ny + R ? j - a.ny : j);
+        ks = k < R ? k + a.nz : (k >= a.nz + R ? k - a.nz : k);
+    }
+    else {  // proper modulo: an extent smaller than the halo width wraps more than once
+        is = ((i - R) % a.nx + a.nx) % a.nx + R;
+        js = ((j - R) % a.ny + a.ny) % a.ny + R;
+        ks = ((k - R) % a.nz + a.nz) % a.nz + R;
+    }
+    const long long pl = (long long)plane;
+    const long long dst = (long long)i + (long long)a.mx * j + pl * k;
+    const long long src = (long long)is + (long long)a.mx * js + pl * ks;
     f[dst] = f[src];
 }
 
@@ -91,7 +96,13 @@ void launch_periodic_fill(T* const* fields, int nfields, int nx, int ny, int nz,
     a.nz = nz;
     a.mx = nx + 2 * R;
     a.my = ny + 2 * R;
-    a.halo_count = 6LL * a.mx * a.my + (long long)nz * (6LL * a.mx + 6LL * ny);
+    const long long hc = 6LL * a.mx * a.my + (long long)nz * (6LL * a.mx + 6LL * ny);
+    if (hc >= (1LL << 32) || 6LL * a.mx * a.my >= (1LL << 32)) {
+        fprintf(stderr, "astaroth_amd: periodic fill: %lld halo cells per field exceed 2^32\n", hc);
+        abort();
+    }
+    a.halo_count = (unsigned)hc;
+    a.wide = nx >= R && ny >= R && nz >= R;
     const dim3 tpb(256, 1, 1);
     const dim3 bpg((unsigned)((a.halo_count + 255) / 256), (unsigned)nfields, 1);
     hipLaunchKernelGGL(periodic_fill_kernel<T>, bpg, tpb, 0, s, a);

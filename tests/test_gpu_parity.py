@@ -46,7 +46,8 @@ def lib(request, require_gpu):
 
 
 def test_periodic_fill_bitexact(lib):
-    for n in [(16, 16, 16), (13, 9, 11)]:
+    # (70, 40, 33): many blocks of the index decomposition
+    for n in [(16, 16, 16), (13, 9, 11), (70, 40, 33)]:
         say(f"case periodic fill {lib.precision} n={n}")
         info = lib.config(n)
         mesh = ac.HostMesh(lib, info)
@@ -59,6 +60,27 @@ def test_periodic_fill_bitexact(lib):
         ref = mesh.data.copy()
         O.periodic_fill(ref, O.config_from_info(info))
         np.testing.assert_array_equal(out.data, ref)
+
+
+def test_periodic_fill_narrow(lib):
+    """Extents below the halo width (2 < 3): the image wraps more than once. The check is the
+    periodic map itself, source index modulo n, in numpy: the reference's device fill takes a
+    modulo too (ref acc-runtime/built-in/kernels.h:17-41; its y and z wraps add nlocal.x, which
+    shifts the image unless nx is a multiple of ny and nz), and the host fill the oracle
+    restates does not wrap more than once."""
+    n = (2, 5, 3)
+    say(f"case periodic fill narrow {lib.precision} n={n}")
+    info = lib.config(n)
+    mesh = ac.HostMesh(lib, info)
+    mesh.data[...] = np.random.default_rng(6).random(mesh.data.shape)
+    with ac.Device(lib, info) as dev:
+        dev.load_mesh(mesh)
+        dev.periodic()
+        out = ac.HostMesh(lib, info)
+        dev.store_mesh(out)
+    src = [(np.arange(e + 6) - 3) % e + 3 for e in n]  # x, y, z
+    ref = mesh.data[:, src[2]][:, :, src[1]][:, :, :, src[0]]
+    np.testing.assert_array_equal(out.data, ref)
 
 
 def run_device_steps(lib, info, data, dt, steps):
