@@ -25,7 +25,8 @@ FIELD_NAMES = ["VTXBUF_LNRHO", "VTXBUF_UUX", "VTXBUF_UUY", "VTXBUF_UUZ", "VTXBUF
                "VTXBUF_AY", "VTXBUF_AZ", "VTXBUF_ENTROPY"]
 
 INT_PARAMS = ["AC_max_steps", "AC_save_steps", "AC_start_step", "AC_kahan_bits",
-              "AC_solve_tile_x", "AC_solve_tile_y", "AC_solve_zchunk", "AC_forcing_period_steps"]
+              "AC_solve_tile_x", "AC_solve_tile_y", "AC_solve_zchunk", "AC_forcing_period_steps",
+              "AC_solve_kernel"]
 INT3_PARAMS = ["AC_ngrid", "AC_nlocal", "AC_mlocal", "AC_mgrid", "AC_nmin", "AC_nlocal_max",
                "AC_ngrid_max", "AC_multigpu_offset"]
 REAL_PARAMS = ["AC_cdt", "AC_cdtv", "AC_cdts", "AC_nu_visc", "AC_cs_sound", "AC_zeta", "AC_eta",
@@ -76,7 +77,8 @@ EXPORTED = [
     "acDeviceSynchronizeStream", "acDeviceSwapBuffer", "acDeviceSwapBuffers",
     "acDeviceLoadScalarUniform", "acDeviceLoadVectorUniform", "acDeviceLoadIntUniform",
     "acDeviceLoadBoolUniform", "acDeviceLoadInt3Uniform", "acDeviceStoreScalarUniform",
-    "acDeviceStoreVectorUniform", "acDeviceLoadMeshInfo", "acDeviceLoadVertexBufferWithOffset",
+    "acDeviceStoreVectorUniform", "acDeviceLoadMeshInfo", "acDeviceLoadStencilsFromConfig",
+    "acDeviceLoadVertexBufferWithOffset",
     "acDeviceLoadMeshWithOffset", "acDeviceLoadVertexBuffer", "acDeviceLoadMesh",
     "acDeviceSetVertexBuffer", "acDeviceFlushOutputBuffers", "acDeviceStoreVertexBufferWithOffset",
     "acDeviceGetConfig", "acDeviceGetLocalConfig", "acDeviceStoreMeshWithOffset",
@@ -86,7 +88,8 @@ EXPORTED = [
     "acDeviceReduceVec", "acDeviceReduceVecScalNoPostProcessing", "acDeviceReduceVecScal",
     "acDeviceUpdateTimestepCFL", "acDeviceGetTimestep", "acDeviceIntegrateSubstepDeviceDt",
     "acDeviceIntegrateStepsCFL", "acDeviceReduceDiagnostics", "acDeviceAutotune",
-    "acDeviceGetSolveLaunchShape", "acDeviceGetVertexBufferPtrs", "acDeviceSetSolveTiming",
+    "acDeviceGetSolveLaunchShape", "acDeviceGetSolveKernel", "acDeviceGetVertexBufferPtrs",
+    "acDeviceSetSolveTiming",
     "acDeviceGetSolveTiming", "acDeviceGetStreamHandle",
     "acHostUpdateParams", "acSetGridMeshDims", "acSetLocalMeshDims", "acLoadConfig",
     "acSetDefaultConfig", "acHostMeshCreate", "acHostMeshDestroy", "acHostMeshCopyVertexBuffers",
@@ -246,6 +249,7 @@ class AstarothLib:
             "acDeviceStoreScalarUniform": [dev, i, i, P(R)],
             "acDeviceStoreVectorUniform": [dev, i, i, P(R3)],
             "acDeviceLoadMeshInfo": [dev, Info],
+            "acDeviceLoadStencilsFromConfig": [dev, i],
             "acDeviceLoadVertexBufferWithOffset": [dev, i, Mesh, i, Int3, Int3, i],
             "acDeviceLoadMeshWithOffset": [dev, i, Mesh, Int3, Int3, i],
             "acDeviceLoadVertexBuffer": [dev, i, Mesh, i],
@@ -272,6 +276,7 @@ class AstarothLib:
             "acDeviceReduceDiagnostics": [dev, i, P(AcDiagnostics)],
             "acDeviceAutotune": [dev, i],
             "acDeviceGetSolveLaunchShape": [dev, P(i), P(i), P(i)],
+            "acDeviceGetSolveKernel": [dev, P(i)],
             "acDeviceGetVertexBufferPtrs": [dev, i, P(ctypes.c_void_p), P(ctypes.c_void_p),
                                             P(ctypes.c_void_p)],
             "acDeviceSetSolveTiming": [dev, i],

@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 OBJDIR = os.path.join(HERE, "_build")
 
-SOURCES = ["device.hip", "solve_direct.hip", "solve_tiled.hip", "solve_ring.hip",
+SOURCES = ["device.hip", "solve_direct.hip", "solve_tiled.hip", "solve_ring.hip", "solve_sweep.hip",
            "aux_kernels.hip", "ic.hip", "host.cc", "adjoint.hip", "forcing.cc"]
 ARCH = os.environ.get("ASTAROTH_OFFLOAD_ARCH", "gfx950")
 
@@ -31,6 +31,10 @@ HIP_FLAGS = [
 # 16-bit ones are unchanged); the FP64 kernels keep the default scheduler, which is faster for
 # them.
 PRECISION_FLAGS = {"fp32": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"], "fp64": []}
+
+# Per-source flags. The sweep kernel keeps its loop-invariant addresses inside the loop: hoisted by
+# machine LICM they hold registers across the whole z-march, and the kernel then spills.
+SOURCE_FLAGS = {"solve_sweep.hip": ["-mllvm", "-disable-machine-licm"]}
 
 
 def _hipcc():
@@ -61,7 +65,8 @@ def build(verbose=True, jobs=None):
         for s in sources:
             obj = os.path.join(OBJDIR, f"{os.path.splitext(s)[0]}_{prec}.o")
             tasks.append((os.path.join(CSRC, s), obj,
-                          [f"-DAC_DOUBLE_PRECISION={dp}"] + PRECISION_FLAGS[prec]))
+                          [f"-DAC_DOUBLE_PRECISION={dp}"] + PRECISION_FLAGS[prec] +
+                          SOURCE_FLAGS.get(s, [])))
     jobs = jobs or min(8, os.cpu_count() or 4)
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(_compile, tasks))

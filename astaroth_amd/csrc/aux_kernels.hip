@@ -150,6 +150,87 @@ __device__ __forceinline__ double red_map(int map, const T* const* bufs, long lo
     }
 }
 
+// Interior cell number t (x fastest) -> index in an m^3 array, in the index type I: 32-bit
+// unsigned when the interior has fewer than 2^31 cells and the array fewer than 2^32 (every grid
+// up to about 1600^3; use_index32 below), 64-bit otherwise. The grid-stride loops visit the same
+// cells in the same order with either type, so the reductions are bit-identical.
+template <typename I>
+__device__ __forceinline__ I interior_index(I t, I nx, I ny, I mx, I sz)
+{
+    const I i = t % nx;
+    const I r = t / nx;
+    const I j = r % ny;
+    const I k = r / ny;
+    return (i + R) + mx * (j + R) + sz * (k + R);
+}
+
+static bool use_index32(int nx, int ny, int nz)
+{
+    const long long n = (long long)nx * ny * nz;
+    const long long m = (long long)(nx + 2 * R) * (ny + 2 * R) * (nz + 2 * R);
+    return n < (1ll << 31) && m < (1ll << 32);
+}
+
+// The halo cells inside a box [x0, x1) x [y0, y1) x [z0, z1) of the m^3 array (a sub-volume fill,
+// ref src/core/device.cc:998-1017 with the BOUNDCOND_PERIODIC_DEVICE kernel of
+// acc-runtime/built-in/kernels.h:17-41, which skips the interior): one thread per box cell and
+// field; interior cells are left alone, halo cells take their periodic image.
+template <typename T>
+struct FillBoxArgs {
+    T* f[NF];
+    int nx, ny, nz, mx, my;
+    int x0, y0, z0, bx, by;
+    unsigned count;  // box cells per field
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) periodic_fill_box_kernel(FillBoxArgs<T> a)
+{
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.count)
+        return;
+    const unsigned bx = (unsigned)a.bx, by = (unsigned)a.by;
+    const unsigned r = t / bx;
+    const int i = a.x0 + (int)(t - r * bx);
+    const int j = a.y0 + (int)(r % by);
+    const int k = a.z0 + (int)(r / by);
+    const bool inside = i >= R && i < a.nx + R && j >= R && j < a.ny + R && k >= R && k < a.nz + R;
+    if (inside)
+        return;
+    const int is = ((i - R) % a.nx + a.nx) % a.nx + R;
+    const int js = ((j - R) % a.ny + a.ny) % a.ny + R;
+    const int ks = ((k - R) % a.nz + a.nz) % a.nz + R;
+    T* __restrict__ f = a.f[blockIdx.y];
+    const long long pl = (long long)a.mx * a.my;
+    f[(long long)i + (long long)a.mx * j + pl * k] = f[(long long)is + (long long)a.mx * js + pl * ks];
+}
+
+template <typename T>
+void launch_periodic_fill_box(T* const* fields, int nfields, int nx, int ny, int nz, const int* lo,
+                              const int* hi, hipStream_t s)
+{
+    FillBoxArgs<T> a;
+    for (int q = 0; q < NF; ++q)
+        a.f[q] = q < nfields ? fields[q] : nullptr;
+    a.nx = nx;
+    a.ny = ny;
+    a.nz = nz;
+    a.mx = nx + 2 * R;
+    a.my = ny + 2 * R;
+    a.x0 = lo[0];
+    a.y0 = lo[1];
+    a.z0 = lo[2];
+    a.bx = hi[0] - lo[0];
+    a.by = hi[1] - lo[1];
+    const long long cnt = (long long)a.bx * a.by * (hi[2] - lo[2]);
+    if (cnt <= 0)
+        return;
+    a.count = (unsigned)cnt;  // a box of an m^3 array the caller checked (< 2^32 cells)
+    const dim3 tpb(256, 1, 1);
+    const dim3 bpg((unsigned)((cnt + 255) / 256), (unsigned)nfields, 1);
+    hipLaunchKernelGGL(periodic_fill_box_kernel<T>, bpg, tpb, 0, s, a);
+}
+
 template <typename T>
 struct RedArgs {
     const T* bufs[4];
@@ -176,21 +257,16 @@ __device__ __forceinline__ double block_reduce(int op, double v, double* sh)
     return r;
 }
 
-template <typename T>
+template <typename T, typename I>
 __global__ void __launch_bounds__(RED_TPB) reduce_stage1(RedArgs<T> a)
 {
     __shared__ double sh[RED_TPB / 64];
-    const long long n = (long long)a.nx * a.ny * a.nz;
+    const I n = (I)a.nx * (I)a.ny * (I)a.nz;
+    const I mx = (I)a.mx, sz = (I)a.mx * (I)a.my;
     double acc = red_init(a.op);
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-         t += (long long)gridDim.x * blockDim.x) {
-        const int i = (int)(t % a.nx);
-        const long long r = t / a.nx;
-        const int j = (int)(r % a.ny);
-        const int k = (int)(r / a.ny);
-        const long long c = (long long)(i + R) + (long long)a.mx * (j + R) +
-                            (long long)a.mx * a.my * (k + R);
-        acc = red_combine(a.op, acc, red_map(a.map, a.bufs, c));
+    for (I t = (I)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (I)gridDim.x * blockDim.x) {
+        const I c = interior_index<I>(t, (I)a.nx, (I)a.ny, mx, sz);
+        acc = red_combine(a.op, acc, red_map(a.map, a.bufs, (long long)c));
     }
     const double r = block_reduce(a.op, acc, sh);
     if (threadIdx.x == 0)
@@ -238,7 +314,10 @@ void launch_reduce(const T* const* bufs, int map, int op, int nx, int ny, int nz
     int nb = reduce_num_partials(nx, ny, nz);
     if (nb > max_partials)
         nb = max_partials;
-    hipLaunchKernelGGL(reduce_stage1<T>, dim3(nb), dim3(RED_TPB), 0, s, a);
+    if (use_index32(nx, ny, nz))
+        hipLaunchKernelGGL((reduce_stage1<T, unsigned>), dim3(nb), dim3(RED_TPB), 0, s, a);
+    else
+        hipLaunchKernelGGL((reduce_stage1<T, unsigned long long>), dim3(nb), dim3(RED_TPB), 0, s, a);
     hipLaunchKernelGGL(reduce_stage2, dim3(1), dim3(RED_TPB), 0, s, (const double*)partials, nb,
                        op, result_dev);
 }
@@ -266,7 +345,7 @@ __device__ __forceinline__ int diag_op(int slot)
     return r == 0 ? 3 : (r == 1 ? 2 : 1);
 }
 
-template <typename T>
+template <typename T, typename I>
 __global__ void __launch_bounds__(RED_TPB) diagnostics_stage1(DiagArgs<T> a)
 {
     __shared__ double sh[RED_TPB / 64];
@@ -274,15 +353,10 @@ __global__ void __launch_bounds__(RED_TPB) diagnostics_stage1(DiagArgs<T> a)
 #pragma unroll
     for (int q = 0; q < DIAG_SLOTS; ++q)
         acc[q] = red_init(diag_op(q));
-    const long long n = (long long)a.nx * a.ny * a.nz;
-    const long long sy = a.mx, sz = (long long)a.mx * a.my;
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-         t += (long long)gridDim.x * blockDim.x) {
-        const int i = (int)(t % a.nx);
-        const long long r = t / a.nx;
-        const int j = (int)(r % a.ny);
-        const int k = (int)(r / a.ny);
-        const long long c = (long long)(i + R) + sy * (j + R) + sz * (k + R);
+    const I n = (I)a.nx * (I)a.ny * (I)a.nz;
+    const I sy = (I)a.mx, sz = (I)a.mx * (I)a.my;
+    for (I t = (I)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (I)gridDim.x * blockDim.x) {
+        const I c = interior_index<I>(t, (I)a.nx, (I)a.ny, sy, sz);
 #pragma unroll
         for (int q = 0; q < NF; ++q) {
             const double v = a.f[q][c];
@@ -295,14 +369,14 @@ __global__ void __launch_bounds__(RED_TPB) diagnostics_stage1(DiagArgs<T> a)
         acc[3 * NF + 0] += u2;
         acc[3 * NF + 1] = fmax(acc[3 * NF + 1], u2);
         acc[3 * NF + 2] = fmin(acc[3 * NF + 2], u2);
-        const long long st[3] = {1, sy, sz};
+        const I st[3] = {1, sy, sz};
         double g[3][3];
 #pragma unroll
         for (int comp = 0; comp < 3; ++comp) {
             const T* f = a.f[4 + comp];
 #pragma unroll
             for (int ax = 0; ax < 3; ++ax) {
-                const long long s = st[ax];
+                const I s = st[ax];
                 const T* kk = a.C.d1[ax];
                 g[comp][ax] = (double)(kk[1] * (f[c + s] - f[c - s]) + kk[2] * (f[c + 2 * s] - f[c - 2 * s]) +
                                        kk[3] * (f[c + 3 * s] - f[c - 3 * s]));
@@ -359,7 +433,11 @@ void launch_diagnostics(const T* const* fields, const Coeffs<T>& C, int nx, int 
     int nb = reduce_num_partials(nx, ny, nz);
     if (nb * DIAG_SLOTS > max_partials)
         nb = max_partials / DIAG_SLOTS;
-    hipLaunchKernelGGL(diagnostics_stage1<T>, dim3(nb), dim3(RED_TPB), 0, s, a);
+    if (use_index32(nx, ny, nz))
+        hipLaunchKernelGGL((diagnostics_stage1<T, unsigned>), dim3(nb), dim3(RED_TPB), 0, s, a);
+    else
+        hipLaunchKernelGGL((diagnostics_stage1<T, unsigned long long>), dim3(nb), dim3(RED_TPB), 0,
+                           s, a);
     hipLaunchKernelGGL(diagnostics_stage2, dim3(1), dim3(RED_TPB), 0, s, (const double*)partials,
                        nb, out_dev);
 }
@@ -372,19 +450,14 @@ struct UmaxArgs {
     unsigned long long* umax2;
 };
 
-template <typename T>
+template <typename T, typename I>
 __global__ void __launch_bounds__(RED_TPB) umax2_kernel(UmaxArgs<T> a)
 {
-    const long long n = (long long)a.nx * a.ny * a.nz;
+    const I n = (I)a.nx * (I)a.ny * (I)a.nz;
+    const I mx = (I)a.mx, sz = (I)a.mx * (I)a.my;
     double m = 0.0;
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-         t += (long long)gridDim.x * blockDim.x) {
-        const int i = (int)(t % a.nx);
-        const long long r = t / a.nx;
-        const int j = (int)(r % a.ny);
-        const int k = (int)(r / a.ny);
-        const long long c = (long long)(i + R) + (long long)a.mx * (j + R) +
-                            (long long)a.mx * a.my * (k + R);
+    for (I t = (I)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (I)gridDim.x * blockDim.x) {
+        const I c = interior_index<I>(t, (I)a.nx, (I)a.ny, mx, sz);
         const double x = a.u[0][c], y = a.u[1][c], z = a.u[2][c];
         m = fmax(m, x * x + y * y + z * z);
     }
@@ -405,7 +478,10 @@ void launch_umax2(const T* const* uu, int nx, int ny, int nz, unsigned long long
     a.my = ny + 2 * R;
     a.umax2 = umax2;
     const int nb = reduce_num_partials(nx, ny, nz);
-    hipLaunchKernelGGL(umax2_kernel<T>, dim3(nb), dim3(RED_TPB), 0, s, a);
+    if (use_index32(nx, ny, nz))
+        hipLaunchKernelGGL((umax2_kernel<T, unsigned>), dim3(nb), dim3(RED_TPB), 0, s, a);
+    else
+        hipLaunchKernelGGL((umax2_kernel<T, unsigned long long>), dim3(nb), dim3(RED_TPB), 0, s, a);
 }
 
 // ---------------------------------------------------------------- reference timestep inputs
@@ -421,20 +497,15 @@ struct CflArgs {
     unsigned long long* out;  // pair
 };
 
-template <typename T>
+template <typename T, typename I>
 __global__ void __launch_bounds__(RED_TPB) cfl_state_kernel(CflArgs<T> a)
 {
-    const long long n = (long long)a.nx * a.ny * a.nz;
-    const long long sy = a.mx, sz = (long long)a.mx * a.my;
+    const I n = (I)a.nx * (I)a.ny * (I)a.nz;
+    const I sy = (I)a.mx, sz = (I)a.mx * (I)a.my;
     double mx_ = 0.0, my_ = 0.0;
-    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-         t += (long long)gridDim.x * blockDim.x) {
-        const int i = (int)(t % a.nx);
-        const long long r = t / a.nx;
-        const int j = (int)(r % a.ny);
-        const int k = (int)(r / a.ny);
-        const long long c = (long long)(i + R) + sy * (j + R) + sz * (k + R);
-        const long long st[3] = {1, sy, sz};
+    for (I t = (I)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (I)gridDim.x * blockDim.x) {
+        const I c = interior_index<I>(t, (I)a.nx, (I)a.ny, sy, sz);
+        const I st[3] = {1, sy, sz};
         CellDerivs<T> d;
         d.lr = a.f[0][c];
 #pragma unroll
@@ -443,7 +514,7 @@ __global__ void __launch_bounds__(RED_TPB) cfl_state_kernel(CflArgs<T> a)
             const T* f = a.f[4 + q];
 #pragma unroll
             for (int ax = 0; ax < 3; ++ax) {
-                const long long s = st[ax];
+                const I s = st[ax];
                 const T* kk = a.C.d1[ax];
                 d.ga[q][ax] = kk[1] * (f[c + s] - f[c - s]) + kk[2] * (f[c + 2 * s] - f[c - 2 * s]) +
                               kk[3] * (f[c + 3 * s] - f[c - 3 * s]);
@@ -477,7 +548,11 @@ void launch_cfl_state(const T* const* fields, const Coeffs<T>& C, int nx, int ny
     a.out = out;
     const long long n = (long long)nx * ny * nz;
     const int nb = (int)std::min<long long>((n + RED_TPB - 1) / RED_TPB, 4096);
-    hipLaunchKernelGGL(cfl_state_kernel<T>, dim3(nb), dim3(RED_TPB), 0, s, a);
+    if (use_index32(nx, ny, nz))
+        hipLaunchKernelGGL((cfl_state_kernel<T, unsigned>), dim3(nb), dim3(RED_TPB), 0, s, a);
+    else
+        hipLaunchKernelGGL((cfl_state_kernel<T, unsigned long long>), dim3(nb), dim3(RED_TPB), 0, s,
+                           a);
 }
 
 template void launch_cfl_state<AcRealT>(const AcRealT* const*, const Coeffs<AcRealT>&, int, int, int,
@@ -593,6 +668,8 @@ void launch_zero_bytes(void* dst, size_t nbytes, hipStream_t s)
 }
 
 template void launch_periodic_fill<AcRealT>(AcRealT* const*, int, int, int, int, hipStream_t);
+template void launch_periodic_fill_box<AcRealT>(AcRealT* const*, int, int, int, int, const int*,
+                                                  const int*, hipStream_t);
 template void launch_reduce<AcRealT>(const AcRealT* const*, int, int, int, int, int, double*, int,
                                      double*, hipStream_t);
 template void launch_diagnostics<AcRealT>(const AcRealT* const*, const Coeffs<AcRealT>&, int, int,

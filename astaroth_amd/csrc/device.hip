@@ -47,6 +47,7 @@ constexpr int kTimingSlots = 4096;
 constexpr int kDiagSlots = DIAG_SLOTS;
 }  // namespace
 
+
 struct device_s {
     int id;
     AcMeshInfo local_config;
@@ -75,6 +76,8 @@ struct device_s {
     bool periodic_fused;          // substeps read the periodic images (no halo fill before them)
     bool forcing_time_acc;        // forcing start time = AC_current_time + time_acc - dt
     double* host_pinned;          // staging for small D2H reads
+    // stencil coefficients (grid spacing folded in), set by acDeviceLoadStencilsFromConfig
+    Coeffs<AcReal> C;
     // launch shape
     TiledShape shape;
     bool shape_tuned;
@@ -184,7 +187,7 @@ static void fill_solve_args(Device d, SolveArgs<AcReal>& a, int step, AcReal dt)
     a.mz = n.z + 2 * R;
     a.sz = (long long)a.mx * a.my;
     a.fstride = (long long)d->count;
-    make_coeffs(info, a.C);
+    a.C = d->C;
     make_phys(info, a.P);
     const AcReal alpha[3] = {(AcReal)0.0, (AcReal)(-5.0 / 9.0), (AcReal)(-153.0 / 128.0)};
     const AcReal beta[3] = {(AcReal)(1.0 / 3.0), (AcReal)(15.0 / 16.0), (AcReal)(8.0 / 15.0)};
@@ -202,6 +205,37 @@ static void fill_solve_args(Device d, SolveArgs<AcReal>& a, int step, AcReal dt)
     a.wrap = d->periodic_fused ? 1 : 0;
 }
 
+// Launch shapes: kind 0 runs the delay-line (or ring) kernel of solve_tiled.hip, kind 1 the
+// unrolled sweep of solve_sweep.hip; tx == 0 is the direct kernel.
+static bool shape_fits(TiledShape s, int3 n)
+{
+    return s.kind == 1 ? sweep_shape_fits(s, n.x, n.y, n.z) : tiled_shape_fits(s, n.x, n.y, n.z);
+}
+
+// Whether the kernel that runs for shape s reads the periodic images itself (honours a.wrap).
+static bool shape_reads_periodic(const SolveArgs<AcReal>& a, int kb, TiledShape s)
+{
+    if (s.tx <= 0)
+        return false;
+    return s.kind == 1 ? sweep_applies<AcReal>(a, kb, s) : tiled_reads_periodic<AcReal>(a, kb, s);
+}
+
+static bool launch_shape(const SolveArgs<AcReal>& a, bool upwd, bool entropy, int kb, TiledShape s,
+                         hipStream_t st)
+{
+    if (s.tx <= 0)
+        return false;
+    return s.kind == 1 ? launch_solve_sweep<AcReal>(a, upwd, entropy, kb, s, st)
+                       : launch_solve_tiled<AcReal>(a, upwd, entropy, kb, s, st);
+}
+
+// The shape the configuration pins (AC_solve_tile_x > 0), or the direct kernel's (tx < 0).
+static TiledShape config_shape(const AcMeshInfo& c)
+{
+    return TiledShape{c.int_params[AC_solve_tile_x], c.int_params[AC_solve_tile_y],
+                      c.int_params[AC_solve_zchunk], c.int_params[AC_solve_kernel] == 1 ? 1 : 0};
+}
+
 static AcResult launch_solve(Device d, const SolveArgs<AcReal>& a_in, hipStream_t s)
 {
     const AcMeshInfo& info = d->local_config;
@@ -209,7 +243,7 @@ static AcResult launch_solve(Device d, const SolveArgs<AcReal>& a_in, hipStream_
     const bool entropy = info.bool_params[AC_lentropy] != 0;
     const int kb = d->kahan_bits;
     SolveArgs<AcReal> a = a_in;
-    if (a.wrap && !(d->shape.tx > 0 && tiled_reads_periodic<AcReal>(a, kb, d->shape))) {
+    if (a.wrap && !shape_reads_periodic(a, kb, d->shape)) {
         // the kernel that runs reads the halos: fill them first
         const int3 n = info.int3_params[AC_nlocal];
         launch_periodic_fill<AcReal>(d->in, NF, n.x, n.y, n.z, s);
@@ -218,10 +252,7 @@ static AcResult launch_solve(Device d, const SolveArgs<AcReal>& a_in, hipStream_
     const bool timed = d->timing && d->events_created && d->timing_used < kTimingSlots;
     if (timed)
         AC_HIP_CHECK(hipEventRecord(d->ev_start[d->timing_used], s));
-    bool done = false;
-    if (d->shape.tx > 0)
-        done = launch_solve_tiled<AcReal>(a, upwd, entropy, kb, d->shape, s);
-    if (!done)
+    if (!launch_shape(a, upwd, entropy, kb, d->shape, s))
         launch_solve_direct<AcReal>(a, upwd, entropy, kb, s);
     AC_HIP_CHECK(hipGetLastError());
     if (timed) {
@@ -318,6 +349,7 @@ AcResult acDeviceCreate(const int id, const AcMeshInfo device_config, Device* de
         return fail("acDeviceCreate: AC_kahan_bits must be 0, 8, 16 or 32");
     }
     d->kahan_bits = kb;
+    make_coeffs(d->local_config, d->C);
 
     hipStream_t s0 = get_stream(d, STREAM_0);
     if (!s0) {
@@ -367,13 +399,12 @@ AcResult acDeviceCreate(const int id, const AcMeshInfo device_config, Device* de
     launch_zero_bytes(d->time_acc, sizeof(double), s0);
     AC_HIP_CHECK(hipHostMalloc((void**)&d->host_pinned, sizeof(double) * (2 + kDiagSlots)));
 
-    // AC_solve_tile_x > 0 pins a tiled shape, < 0 pins the direct kernel, 0 autotunes
-    d->shape.tx = d->local_config.int_params[AC_solve_tile_x];
-    d->shape.ty = d->local_config.int_params[AC_solve_tile_y];
-    d->shape.zchunk = d->local_config.int_params[AC_solve_zchunk];
+    // AC_solve_tile_x > 0 pins a tiled shape (AC_solve_kernel selects the kernel), < 0 pins the
+    // direct kernel, 0 autotunes
+    d->shape = config_shape(d->local_config);
     d->shape_tuned = d->shape.tx != 0;
-    if (d->shape.tx < 0 || (d->shape.tx > 0 && !tiled_shape_fits(d->shape, n.x, n.y, n.z)))
-        d->shape = TiledShape{0, 0, 0};
+    if (d->shape.tx < 0 || (d->shape.tx > 0 && !shape_fits(d->shape, n)))
+        d->shape = TiledShape{0, 0, 0, 0};
     AC_HIP_CHECK(hipGetLastError());
     AC_HIP_CHECK(hipStreamSynchronize(s0));
     *device_handle = d;
@@ -431,8 +462,9 @@ AcResult acDevicePrintInfo(const Device d)
     const int3 n = d->local_config.int3_params[AC_nlocal];
     printf("Device %d: %s (%s), %d CUs, %.1f GiB\n", d->id, p.name, p.gcnArchName,
            p.multiProcessorCount, p.totalGlobalMem / 1073741824.0);
-    printf("  grid %d x %d x %d, AcReal %zu bytes, kahan bits %d, solve shape %d x %d x %d\n",
-           n.x, n.y, n.z, sizeof(AcReal), d->kahan_bits, d->shape.tx, d->shape.ty, d->shape.zchunk);
+    printf("  grid %d x %d x %d, AcReal %zu bytes, kahan bits %d, solve shape %d x %d x %d (%s)\n",
+           n.x, n.y, n.z, sizeof(AcReal), d->kahan_bits, d->shape.tx, d->shape.ty, d->shape.zchunk,
+           d->shape.tx <= 0 ? "direct" : (d->shape.kind == 1 ? "sweep" : "delay-line"));
     return AC_SUCCESS;
 }
 
@@ -494,17 +526,16 @@ AcResult acDeviceLoadIntUniform(const Device d, const Stream, const AcIntParam p
     if (param == AC_kahan_bits && value != d->kahan_bits)
         return fail("acDeviceLoadIntUniform: AC_kahan_bits is fixed at acDeviceCreate");
     d->local_config.int_params[param] = value;
-    if (param == AC_solve_tile_x || param == AC_solve_tile_y || param == AC_solve_zchunk) {
+    if (param == AC_solve_tile_x || param == AC_solve_tile_y || param == AC_solve_zchunk ||
+        param == AC_solve_kernel) {
         const int3 n = d->local_config.int3_params[AC_nlocal];
-        TiledShape s{d->local_config.int_params[AC_solve_tile_x],
-                     d->local_config.int_params[AC_solve_tile_y],
-                     d->local_config.int_params[AC_solve_zchunk]};
-        if (s.tx > 0 && tiled_shape_fits(s, n.x, n.y, n.z)) {
+        const TiledShape s = config_shape(d->local_config);
+        if (s.tx > 0 && shape_fits(s, n)) {
             d->shape = s;
             d->shape_tuned = true;
         }
         else if (s.tx <= 0) {
-            d->shape = TiledShape{0, 0, 0};
+            d->shape = TiledShape{0, 0, 0, 0};
             d->shape_tuned = s.tx < 0;
         }
         else {
@@ -565,6 +596,21 @@ AcResult acDeviceLoadMeshInfo(const Device d, const AcMeshInfo config)
     }
     c.int_params[AC_kahan_bits] = d->kahan_bits;
     d->local_config = c;
+    return AC_SUCCESS;
+}
+
+// ref include/astaroth_device.h:229, src/core/device.cc:447-473: evaluates the stencil
+// coefficients from the device's configuration (AC_ds) and makes them the ones every later
+// substep, reduction and diagnostic uses. acDeviceCreate does this once; a caller that changes
+// the grid spacing afterwards (acDeviceLoadVectorUniform, acDeviceLoadMeshInfo) calls it again,
+// as with the reference, whose kernels keep the coefficients last uploaded.
+AcResult acDeviceLoadStencilsFromConfig(const Device d, const Stream)
+{
+    const AcReal3 ds = d->local_config.real3_params[AC_ds];
+    if (!(std::isfinite((double)ds.x) && std::isfinite((double)ds.y) && std::isfinite((double)ds.z) &&
+          ds.x > 0 && ds.y > 0 && ds.z > 0))
+        return fail("acDeviceLoadStencilsFromConfig: AC_ds must be finite and positive");
+    make_coeffs(d->local_config, d->C);
     return AC_SUCCESS;
 }
 
@@ -805,38 +851,65 @@ AcResult acDeviceIntegrateSubstep(const Device d, const Stream stream, const int
     return r;
 }
 
+// The box [start, end) of the m^3 arrays as int bounds; false if it leaves [0, mlocal).
+static bool fill_box(Device d, const Volume start, const Volume end, int lo[3], int hi[3])
+{
+    const int3 m = d->local_config.int3_params[AC_mlocal];
+    const size_t s3[3] = {start.x, start.y, start.z}, e3[3] = {end.x, end.y, end.z};
+    const int m3[3] = {m.x, m.y, m.z};
+    for (int a = 0; a < 3; ++a) {
+        if (s3[a] > e3[a] || e3[a] > (size_t)m3[a])
+            return false;
+        lo[a] = (int)s3[a];
+        hi[a] = (int)e3[a];
+    }
+    return true;
+}
+
+// Fills the halo cells inside [start, end) of the given fields: the full volume [0, mlocal) in
+// the one-launch fill over halo cells only, any other box in the box kernel (ref
+// src/core/device.cc:998-1017, acc-runtime/built-in/kernels.h:17-41).
+static AcResult periodic_fill(Device d, const Stream stream, AcReal* const* f, int nf,
+                              const Volume start, const Volume end, const char* who)
+{
+    int lo[3], hi[3];
+    if (!fill_box(d, start, end, lo, hi)) {
+        fprintf(stderr, "astaroth_amd: %s: start/end must satisfy start <= end <= mlocal\n", who);
+        return AC_FAILURE;
+    }
+    AC_HIP_CHECK(hipSetDevice(d->id));
+    const int3 n = d->local_config.int3_params[AC_nlocal];
+    const int3 m = d->local_config.int3_params[AC_mlocal];
+    hipStream_t s = get_stream(d, stream);
+    if (lo[0] == 0 && lo[1] == 0 && lo[2] == 0 && hi[0] == m.x && hi[1] == m.y && hi[2] == m.z)
+        launch_periodic_fill<AcReal>(f, nf, n.x, n.y, n.z, s);
+    else
+        launch_periodic_fill_box<AcReal>(f, nf, n.x, n.y, n.z, lo, hi, s);
+    AC_HIP_CHECK(hipGetLastError());
+    return AC_SUCCESS;
+}
+
 AcResult acDevicePeriodicBoundcondStep(const Device d, const Stream stream,
                                        const VertexBufferHandle h, const Volume start,
                                        const Volume end)
 {
     if ((int)h < 0 || (int)h >= NF)
         return fail("acDevicePeriodicBoundcondStep: invalid handle");
-    const int3 m = d->local_config.int3_params[AC_mlocal];
-    if (start.x != 0 || start.y != 0 || start.z != 0 || !volume_eq(end, m))
-        return fail("acDevicePeriodicBoundcondStep: only the full volume [0, mlocal) is supported");
-    AC_HIP_CHECK(hipSetDevice(d->id));
-    const int3 n = d->local_config.int3_params[AC_nlocal];
     AcReal* f[1] = {d->in[h]};
-    launch_periodic_fill<AcReal>(f, 1, n.x, n.y, n.z, get_stream(d, stream));
-    AC_HIP_CHECK(hipGetLastError());
-    return AC_SUCCESS;
+    return periodic_fill(d, stream, f, 1, start, end, "acDevicePeriodicBoundcondStep");
 }
 
 AcResult acDevicePeriodicBoundconds(const Device d, const Stream stream, const Volume start,
                                     const Volume end)
 {
-    const int3 m = d->local_config.int3_params[AC_mlocal];
-    if (start.x != 0 || start.y != 0 || start.z != 0 || !volume_eq(end, m))
-        return fail("acDevicePeriodicBoundconds: only the full volume [0, mlocal) is supported");
-    AC_HIP_CHECK(hipSetDevice(d->id));
-    const int3 n = d->local_config.int3_params[AC_nlocal];
-    launch_periodic_fill<AcReal>(d->in, NF, n.x, n.y, n.z, get_stream(d, stream));
-    AC_HIP_CHECK(hipGetLastError());
-    return AC_SUCCESS;
+    return periodic_fill(d, stream, d->in, NF, start, end, "acDevicePeriodicBoundconds");
 }
 
-static AcResult reduce_common(Device d, const Stream stream, const AcReduction r, int map,
-                              const VertexBufferHandle* h, int nh, bool post, AcReal* result)
+// The two-stage reduction with its partial results in `partials` (max_partials of them) and the
+// final value in *result_dev, both device memory.
+static AcResult reduce_into(Device d, const Stream stream, const AcReduction r, int map,
+                            const VertexBufferHandle* h, int nh, bool post, double* partials,
+                            int max_partials, double* result_dev, AcReal* result)
 {
     for (int q = 0; q < nh; ++q)
         if ((int)h[q] < 0 || (int)h[q] >= NF)
@@ -849,16 +922,22 @@ static AcResult reduce_common(Device d, const Stream stream, const AcReduction r
     for (int q = 0; q < nh; ++q)
         bufs[q] = d->in[h[q]];
     const int3 n = d->local_config.int3_params[AC_nlocal];
-    launch_reduce<AcReal>(bufs, (int)map, (int)r.reduce_op, n.x, n.y, n.z, d->partials,
-                          d->max_partials, d->dres, s);
+    launch_reduce<AcReal>(bufs, (int)map, (int)r.reduce_op, n.x, n.y, n.z, partials, max_partials,
+                          result_dev, s);
     AC_HIP_CHECK(hipGetLastError());
-    AC_HIP_CHECK(hipMemcpyAsync(d->host_pinned, d->dres, sizeof(double), hipMemcpyDeviceToHost, s));
+    AC_HIP_CHECK(hipMemcpyAsync(d->host_pinned, result_dev, sizeof(double), hipMemcpyDeviceToHost, s));
     AC_HIP_CHECK(hipStreamSynchronize(s));
     double v = d->host_pinned[0];
     if (post && r.post_processing_op == AC_RMS)
         v = sqrt(v / ((double)n.x * n.y * n.z));
     *result = (AcReal)v;
     return AC_SUCCESS;
+}
+
+static AcResult reduce_common(Device d, const Stream stream, const AcReduction r, int map,
+                              const VertexBufferHandle* h, int nh, bool post, AcReal* result)
+{
+    return reduce_into(d, stream, r, map, h, nh, post, d->partials, d->max_partials, d->dres, result);
 }
 
 AcResult acDeviceReduceScalNoPostProcessing(const Device d, const Stream stream,
@@ -933,13 +1012,11 @@ AcResult acDeviceUpdateTimestepCFL(const Device d, const Stream stream, const Ac
         launch_umax2<AcReal>(uu, n.x, n.y, n.z, slot, s);
     }
     else {
-        Coeffs<AcReal> C;
-        make_coeffs(d->local_config, C);
         const AcReal* f[NF];
         for (int q = 0; q < NF; ++q)
             f[q] = d->in[q];
         const double cs = (double)d->local_config.real_params[AC_cs_sound];
-        launch_cfl_state<AcReal>(f, C, n.x, n.y, n.z, d->cfl_mode, cs * cs, slot, s);
+        launch_cfl_state<AcReal>(f, d->C, n.x, n.y, n.z, d->cfl_mode, cs * cs, slot, s);
     }
     AC_HIP_CHECK(hipGetLastError());
     d->cfl_courant = (double)courant;
@@ -993,13 +1070,11 @@ AcResult acDeviceIntegrateSubstepDeviceDt(const Device d, const Stream stream,
             // The reference modes reduce |u| and vA of the state entering this substep: one
             // pass over it right before the solve, on the same stream (fusing it into the solve
             // would cost the solve kernels registers they need for two waves per SIMD).
-            Coeffs<AcReal> C;
-            make_coeffs(d->local_config, C);
             const AcReal* f[NF];
             for (int q = 0; q < NF; ++q)
                 f[q] = d->in[q];
             const int3 n = d->local_config.int3_params[AC_nlocal];
-            launch_cfl_state<AcReal>(f, C, n.x, n.y, n.z, d->cfl_mode, a.cfl_cs2, next, s);
+            launch_cfl_state<AcReal>(f, d->C, n.x, n.y, n.z, d->cfl_mode, a.cfl_cs2, next, s);
         }
     }
     const AcResult r = launch_solve(d, a, s);
@@ -1057,12 +1132,10 @@ AcResult acDeviceReduceDiagnostics(const Device d, const Stream stream, AcDiagno
     AC_HIP_CHECK(hipSetDevice(d->id));
     hipStream_t s = get_stream(d, stream);
     const int3 n = d->local_config.int3_params[AC_nlocal];
-    Coeffs<AcReal> C;
-    make_coeffs(d->local_config, C);
     const AcReal* f[NF];
     for (int q = 0; q < NF; ++q)
         f[q] = d->in[q];
-    launch_diagnostics<AcReal>(f, C, n.x, n.y, n.z, d->partials, d->max_partials, d->dres + 1, s);
+    launch_diagnostics<AcReal>(f, d->C, n.x, n.y, n.z, d->partials, d->max_partials, d->dres + 1, s);
     AC_HIP_CHECK(hipGetLastError());
     AC_HIP_CHECK(hipMemcpyAsync(d->host_pinned, d->dres + 1, sizeof(double) * kDiagSlots,
                                 hipMemcpyDeviceToHost, s));
@@ -1150,13 +1223,23 @@ AcResult acDeviceGetSolveLaunchShape(const Device d, int* tile_x, int* tile_y, i
     return AC_SUCCESS;
 }
 
+AcResult acDeviceGetSolveKernel(const Device d, int* kind)
+{
+    *kind = d->shape.tx <= 0 ? -1 : d->shape.kind;
+    return AC_SUCCESS;
+}
+
 }  // extern "C"
 
 // ------------------------------------------------------------------ autotuner
-// Times every candidate shape of the tiled kernel on the current grid (dry run into the out and
-// w buffers from a copy of the state, so the caller's data is untouched) and keeps the fastest.
-// Candidates that do not fit the grid are skipped. The choice can be cached in the file named
-// by ASTAROTH_AUTOTUNE_CACHE (lines "nx ny nz kahan upwd entropy tx ty zchunk").
+// Times every candidate launch (the direct kernel, the delay-line shapes and the sweep shapes) on
+// the current grid with the device's own compensation width, physics switches and periodic-image
+// reads, at substep 1 (reads and writes w), and keeps the fastest. The trial launches read the
+// real state and write everything -- f', w and the compensation records -- into the out slab,
+// which holds nothing between substeps: no allocation, and the state in `in`, `w` and the
+// compensation slab is untouched. The choice can be cached in the file named by
+// ASTAROTH_AUTOTUNE_CACHE (lines "nx ny nz kahan upwd entropy realsize tx ty zchunk kind");
+// ASTAROTH_AUTOTUNE_VERBOSE=1 prints each candidate's time to stderr.
 static bool read_cache(Device d, TiledShape* out)
 {
     const char* path = getenv("ASTAROTH_AUTOTUNE_CACHE");
@@ -1166,15 +1249,18 @@ static bool read_cache(Device d, TiledShape* out)
     if (!fp)
         return false;
     const int3 n = d->local_config.int3_params[AC_nlocal];
-    int v[9];
+    char line[256];
     bool found = false;
-    while (fscanf(fp, "%d %d %d %d %d %d %d %d %d", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6],
-                  &v[7], &v[8]) == 9) {
+    while (fgets(line, sizeof(line), fp)) {
+        int v[11];
+        if (sscanf(line, "%d %d %d %d %d %d %d %d %d %d %d", &v[0], &v[1], &v[2], &v[3], &v[4],
+                   &v[5], &v[6], &v[7], &v[8], &v[9], &v[10]) != 11)
+            continue;
         if (v[0] == n.x && v[1] == n.y && v[2] == n.z && v[3] == d->kahan_bits &&
             v[4] == d->local_config.bool_params[AC_lupwd] &&
-            v[5] == d->local_config.bool_params[AC_lentropy]) {
-            TiledShape s{v[6], v[7], v[8]};
-            if (s.tx == 0 || tiled_shape_fits(s, n.x, n.y, n.z)) {
+            v[5] == d->local_config.bool_params[AC_lentropy] && v[6] == (int)sizeof(AcReal)) {
+            const TiledShape s{v[7], v[8], v[9], v[10]};
+            if (s.tx == 0 || shape_fits(s, n)) {
                 *out = s;
                 found = true;
             }
@@ -1193,16 +1279,20 @@ static void write_cache(Device d, TiledShape s)
     if (!fp)
         return;
     const int3 n = d->local_config.int3_params[AC_nlocal];
-    fprintf(fp, "%d %d %d %d %d %d %d %d %d\n", n.x, n.y, n.z, d->kahan_bits,
-            d->local_config.bool_params[AC_lupwd], d->local_config.bool_params[AC_lentropy], s.tx,
-            s.ty, s.zchunk);
+    fprintf(fp, "%d %d %d %d %d %d %d %d %d %d %d\n", n.x, n.y, n.z, d->kahan_bits,
+            d->local_config.bool_params[AC_lupwd], d->local_config.bool_params[AC_lentropy],
+            (int)sizeof(AcReal), s.tx, s.ty, s.zchunk, s.kind);
     fclose(fp);
 }
+
+static const TiledShape kSweepShapes[] = {
+    {64, 4, 32, 1}, {64, 4, 64, 1}, {64, 4, 128, 1}, {64, 4, 256, 1},
+};
 
 static AcResult autotune(Device d, hipStream_t s)
 {
     const int3 n = d->local_config.int3_params[AC_nlocal];
-    TiledShape best{0, 0, 0};
+    TiledShape best{0, 0, 0, 0};
     if (read_cache(d, &best)) {
         d->shape = best;
         d->shape_tuned = true;
@@ -1210,22 +1300,42 @@ static AcResult autotune(Device d, hipStream_t s)
     }
     const bool upwd = d->local_config.bool_params[AC_lupwd] != 0;
     const bool entropy = d->local_config.bool_params[AC_lentropy] != 0;
-    hipEvent_t e0, e1;
-    AC_HIP_CHECK(hipEventCreate(&e0));
-    AC_HIP_CHECK(hipEventCreate(&e1));
-    // Dry runs read `in` and write `out` and `w` at step 1: w is restored from a copy afterwards,
-    // and Kahan compensation is disabled for the trial launches so it stays untouched.
-    const size_t wbytes = d->count * sizeof(AcReal) * NF;
-    void* wsave = nullptr;
-    AC_HIP_CHECK(hipMalloc(&wsave, wbytes));
-    AC_HIP_CHECK(hipMemcpyAsync(wsave, d->slab_w, wbytes, hipMemcpyDeviceToDevice, s));
+    const int kb = d->kahan_bits;
+    const bool verbose = getenv("ASTAROTH_AUTOTUNE_VERBOSE") != nullptr;
     SolveArgs<AcReal> a;
     fill_solve_args(d, a, 1, (AcReal)1e-9);
+    // Trial outputs alias the out slab: f', w and the compensation records (at most
+    // NF * KB / 8 <= NF * sizeof(float) bytes per cell) all stay inside it. Only a single slab
+    // (no field swapped alone) qualifies; otherwise only the direct kernel is timed, into out.
+    bool one_slab = true;
+    for (int q = 0; q < NF; ++q)
+        one_slab = one_slab && d->out[q] == d->out[0] + (size_t)q * d->count;
+    for (int q = 0; q < NF; ++q) {
+        a.w[q] = a.out[q];
+        a.comp[q] = kb > 0 ? (void*)((char*)d->out[0] + (size_t)q * (kb / 8)) : nullptr;
+    }
+    a.wrap = 1;
+    hipEvent_t e0, e1;
+    AC_HIP_CHECK(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        hipEventDestroy(e0);
+        return fail("autotune: event creation failed");
+    }
     // Each candidate: one warm-up launch, then the best of three timed pairs of launches (the
     // minimum is robust against another process's work or a clock ramp during one sample).
-    auto time_candidate = [&](auto&& launch) -> float {
-        launch();
-        float best = 1e30f;
+    auto time_candidate = [&](TiledShape sh) -> float {
+        auto launch = [&]() -> bool {
+            if (sh.tx <= 0) {
+                SolveArgs<AcReal> ad = a;
+                ad.wrap = 0;  // the direct kernel reads the halos as stored
+                launch_solve_direct<AcReal>(ad, upwd, entropy, kb, s);
+                return true;
+            }
+            return launch_shape(a, upwd, entropy, kb, sh, s);
+        };
+        if (!launch())
+            return 1e30f;
+        float best_ms = 1e30f;
         for (int rep = 0; rep < 3; ++rep) {
             if (hipEventRecord(e0, s) != hipSuccess)
                 return 1e30f;
@@ -1236,30 +1346,39 @@ static AcResult autotune(Device d, hipStream_t s)
             float ms = 0.f;
             if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
                 return 1e30f;
-            best = std::min(best, ms);
+            best_ms = std::min(best_ms, ms);
         }
-        return best;
+        if (verbose)
+            fprintf(stderr, "astaroth_amd autotune: %d^3-ish %dx%dx%d kb=%d kind=%d: %.4f ms per launch\n",
+                    n.x, sh.tx, sh.ty, sh.zchunk, kb, sh.tx <= 0 ? -1 : sh.kind, best_ms / 2);
+        return best_ms;
     };
-    // direct kernel as the baseline candidate
-    float best_ms = time_candidate([&] { launch_solve_direct<AcReal>(a, upwd, entropy, 0, s); });
-    for (int i = 0; i < tiled_num_shapes(); ++i) {
-        const TiledShape sh = tiled_shape(i);
-        if (!tiled_shape_fits(sh, n.x, n.y, n.z))
-            continue;
-        if (!launch_solve_tiled<AcReal>(a, upwd, entropy, 0, sh, s))
-            continue;
-        const float ms = time_candidate([&] { launch_solve_tiled<AcReal>(a, upwd, entropy, 0, sh, s); });
+    std::vector<TiledShape> cands;
+    cands.push_back(TiledShape{0, 0, 0, 0});
+    if (one_slab) {
+        for (int i = 0; i < tiled_num_shapes(); ++i) {
+            const TiledShape sh = tiled_shape(i);
+            // the ring kernels read halos as stored and never win; they stay pinnable only
+            if (sh.tx == 64 && shape_fits(sh, n))
+                cands.push_back(sh);
+        }
+        for (const TiledShape& sh : kSweepShapes)
+            if (shape_fits(sh, n) && sweep_applies<AcReal>(a, kb, sh))
+                cands.push_back(sh);
+    }
+    float best_ms = 1e30f;
+    for (const TiledShape& sh : cands) {
+        const float ms = time_candidate(sh);
         if (ms < best_ms) {
             best_ms = ms;
             best = sh;
         }
     }
-    AC_HIP_CHECK(hipGetLastError());
-    AC_HIP_CHECK(hipMemcpyAsync(d->slab_w, wsave, wbytes, hipMemcpyDeviceToDevice, s));
-    AC_HIP_CHECK(hipStreamSynchronize(s));
-    hipFree(wsave);
+    const hipError_t err = hipGetLastError();
     hipEventDestroy(e0);
     hipEventDestroy(e1);
+    if (err != hipSuccess)
+        return fail("autotune: a trial launch failed");
     d->shape = best;
     d->shape_tuned = true;
     write_cache(d, best);

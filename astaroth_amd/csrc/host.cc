@@ -35,6 +35,7 @@ const NamedParam kParams[] = {
     {"AC_solve_tile_y", 0, AC_solve_tile_y},
     {"AC_solve_zchunk", 0, AC_solve_zchunk},
     {"AC_forcing_period_steps", 0, AC_forcing_period_steps},
+    {"AC_solve_kernel", 0, AC_solve_kernel},
     {"AC_ngrid", 1, AC_ngrid},
     {"AC_nlocal", 1, AC_nlocal},
     {"AC_mlocal", 1, AC_mlocal},
@@ -332,18 +333,21 @@ AcResult acHostUpdateParams(AcMeshInfo* config)
     return AC_SUCCESS;
 }
 
+// This is a single-process build: one device holds the whole grid, so the local mesh is the
+// grid and each setter sets both extents, as the reference's build without MPI does for
+// acSetGridMeshDims (ref src/core/update_params/update_params.cc:173-177, which pushes the grid
+// dims into AC_nlocal). A caller that sets only one of them gets a consistent configuration.
 AcResult acSetGridMeshDims(const size_t nx, const size_t ny, const size_t nz, AcMeshInfo* info)
 {
     info->int3_params[AC_ngrid] = mk3((int)nx, (int)ny, (int)nz);
+    info->int3_params[AC_nlocal] = mk3((int)nx, (int)ny, (int)nz);
     return acHostUpdateParams(info);
 }
 
 AcResult acSetLocalMeshDims(const size_t nx, const size_t ny, const size_t nz, AcMeshInfo* info)
 {
     info->int3_params[AC_nlocal] = mk3((int)nx, (int)ny, (int)nz);
-    const int3 ng = info->int3_params[AC_ngrid];
-    if (ng.x <= 0 || ng.y <= 0 || ng.z <= 0)
-        info->int3_params[AC_ngrid] = mk3((int)nx, (int)ny, (int)nz);
+    info->int3_params[AC_ngrid] = mk3((int)nx, (int)ny, (int)nz);
     return acHostUpdateParams(info);
 }
 

@@ -39,6 +39,7 @@ void launch_solve_direct(const SolveArgs<T>& a, bool upwd, bool entropy, int kah
 // 16 x 16 and 32 x 16 the four-plane ring kernel (solve_ring.hip).
 struct TiledShape {
     int tx, ty, zchunk;
+    int kind;  // 0: delay-line / ring kernel (solve_tiled.hip, solve_ring.hip); 1: sweep (solve_sweep.hip)
 };
 template <typename T>
 bool launch_solve_tiled(const SolveArgs<T>& a, bool upwd, bool entropy, int kahan_bits,
@@ -46,6 +47,15 @@ bool launch_solve_tiled(const SolveArgs<T>& a, bool upwd, bool entropy, int kaha
 template <typename T>
 bool launch_solve_ring(const SolveArgs<T>& a, bool upwd, bool entropy, int kahan_bits,
                        TiledShape shape, hipStream_t stream);
+// The unrolled sweep kernel (FP32, 64 x 4 column blocks, grids with equal spacing along the three
+// axes). Returns false (nothing launched) when it does not apply.
+template <typename T>
+bool launch_solve_sweep(const SolveArgs<T>& a, bool upwd, bool entropy, int kahan_bits,
+                        TiledShape shape, hipStream_t stream);
+bool sweep_shape_fits(TiledShape s, int nx, int ny, int nz);
+// Whether launch_solve_sweep runs the sweep kernel for these arguments (it honours a.wrap).
+template <typename T>
+bool sweep_applies(const SolveArgs<T>& a, int kahan_bits, TiledShape shape);
 // The adjoint's local stage on the tiled machinery: derivative adjoints g (NG fields of m^3,
 // same field stride as the state), lambda_f / lambda_w (NF fields), (nu, eta) block partials.
 template <typename T>
@@ -165,6 +175,10 @@ constexpr int DIAG_SLOTS = 3 * NF + 9;
 // ---- periodic fill: every halo cell of every listed field, one launch ----
 template <typename T>
 void launch_periodic_fill(T* const* fields, int nfields, int nx, int ny, int nz, hipStream_t s);
+// The halo cells of the box [lo, hi) of the m^3 arrays (sub-volume fill); interior cells untouched.
+template <typename T>
+void launch_periodic_fill_box(T* const* fields, int nfields, int nx, int ny, int nz, const int* lo,
+                              const int* hi, hipStream_t s);
 
 // ---- reductions ----
 // map: AcMapKernel value; op: AcReduceOp value. Interior cells only. Result is the raw

@@ -103,13 +103,10 @@ __device__ __forceinline__ void delay_line_body(const SolveArgs<T>& a, const Adj
 
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int tid = tx + TX * ty;
-    // The wave index is uniform across the wave. FP32: kept in a scalar register, so the LDS
+    // The wave index is uniform across the wave: kept in a scalar register, so the LDS
     // destinations of the DMA (M0) are scalar arithmetic (about 70 fewer vector instructions per
-    // plane step). FP64 keeps the per-DMA vector form: its kernels run at 512 VGPRs with spills,
-    // and the scalar form gave wrong results in the 64 x 2 periodic-image instantiation, with no
-    // fault found in the source (the code generation of these FP64 kernels is fragile; see the
-    // WRAP template parameter).
-    const int wave = sizeof(T) == 4 ? __builtin_amdgcn_readfirstlane(tid >> 6) : tid >> 6;
+    // plane step).
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lane = tid & 63;
     const int x0 = tile_x * TX, y0 = tile_y * TY, z0 = chunk * zchunk;
     const long long sy = a.mx, sz = a.sz;
@@ -176,7 +173,11 @@ __device__ __forceinline__ void delay_line_body(const SolveArgs<T>& a, const Adj
     const Coeffs<T>& C = a.C;
     T q[NF][7];
     T ch[NCH][7];
-    PlaneVals<T> dly[4];
+    // The delay line starts zeroed like the queues: every register the loop carries is then
+    // written before it is read on every path (with it left uninitialized, the compiled FP64
+    // kernels copied never-written register pairs through their AGPR spill space each plane
+    // step; see the ISA check in tests/test_isa_invariants.py).
+    PlaneVals<T> dly[4] = {};
     int dslot = 0;  // ring slot of this step's plane p (and of the output plane p - 3)
 #pragma unroll
     for (int f = 0; f < NF; ++f)
@@ -664,6 +665,8 @@ TiledShape tiled_shape(int i) { return kShapes[i]; }
 
 bool tiled_shape_fits(TiledShape s, int nx, int ny, int nz)
 {
+    if (s.kind != 0)
+        return false;
     bool known = false;
     for (const TiledShape& k : kShapes)
         known = known || (k.tx == s.tx && k.ty == s.ty);

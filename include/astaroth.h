@@ -95,6 +95,7 @@ typedef enum {
     AC_solve_tile_y,
     AC_solve_zchunk,
     AC_forcing_period_steps, /* steps between new forcing parameters (config/astaroth.conf:34) */
+    AC_solve_kernel,     /* with a pinned launch shape: 0 = delay-line kernel, 1 = unrolled sweep (FP32) */
     NUM_INT_PARAMS
 } AcIntParam;
 
@@ -357,11 +358,20 @@ AcResult acDeviceStoreMesh(const Device device, const Stream stream, AcMesh* hos
  * After the call the new state is in `out`; the caller swaps, as with the reference. */
 AcResult acDeviceIntegrateSubstep(const Device device, const Stream stream, const int step_number,
                                   const Volume start, const Volume end, const AcReal dt);
-/* ref astaroth_device.h:135, device.cc:998-1007 */
+/* ref astaroth_device.h:135, device.cc:998-1007: the halo cells of one field inside the box
+ * [start, end) (0 <= start <= end <= mlocal) take their periodic images; interior cells are
+ * left alone. */
 AcResult acDevicePeriodicBoundcondStep(const Device device, const Stream stream,
                                        const VertexBufferHandle vtxbuf_handle, const Volume start,
                                        const Volume end);
-/* ref astaroth_device.h:139, device.cc:1009-1017 (here: one launch for all fields) */
+/* ref astaroth_device.h:229, device.cc:447-473: evaluates the stencil coefficients from the
+ * device's AC_ds; they stay in use until the next call (acDeviceCreate makes the first). The
+ * reference's acDeviceLoadStencil(s) / acDeviceStoreStencil take DSL-generated 7x7x7 stencil
+ * tables whose enumeration is generated per DSL program; this build's stencils are the fixed
+ * 6th-order set, so only this form is provided. */
+AcResult acDeviceLoadStencilsFromConfig(const Device device, const Stream stream);
+/* ref astaroth_device.h:139, device.cc:1009-1017: the same for every field, in one launch (the
+ * full volume [0, mlocal) enumerates only its halo cells) */
 AcResult acDevicePeriodicBoundconds(const Device device, const Stream stream, const Volume start,
                                     const Volume end);
 /* ref astaroth_device.h:149 */
@@ -436,6 +446,8 @@ AcResult acDeviceReduceDiagnostics(const Device device, const Stream stream, AcD
 AcResult acDeviceAutotune(const Device device, const Stream stream);
 /* Reports the solve-kernel launch shape in use (tile_x, tile_y, zchunk). */
 AcResult acDeviceGetSolveLaunchShape(const Device device, int* tile_x, int* tile_y, int* zchunk);
+/* Reports which solve kernel that shape runs: -1 direct, 0 delay-line, 1 unrolled sweep. */
+AcResult acDeviceGetSolveKernel(const Device device, int* kind);
 /* Raw device pointers of one field: in (current state), out (next state), w (RK register). */
 AcResult acDeviceGetVertexBufferPtrs(const Device device, const VertexBufferHandle vtxbuf,
                                      AcReal** in, AcReal** out, AcReal** w);

@@ -83,6 +83,73 @@ def test_periodic_fill_narrow(lib):
     np.testing.assert_array_equal(out.data, ref)
 
 
+def test_periodic_fill_subvolume(lib):
+    """acDevicePeriodicBoundconds / acDevicePeriodicBoundcondStep on boxes [start, end) other
+    than the full volume (ref src/core/device.cc:998-1017): halo cells inside the box take their
+    periodic images, every other value is unchanged; an inverted or oversized box is rejected."""
+    n = (13, 9, 11)
+    info = lib.config(n)
+    m = info["AC_mlocal"]
+    rng = np.random.default_rng(8)
+    data = rng.random((ac.NUM_VTXBUF_HANDLES, m[2], m[1], m[0]))
+    full = data.copy()
+    O.periodic_fill(full, O.config_from_info(info))
+    boxes = [((0, 0, 0), (m[0], m[1], 3)), ((2, 1, 0), (9, m[1], m[2])), ((0, 0, 0), (4, 5, 6)),
+             ((5, 5, 5), (7, 7, 7))]
+    for lo, hi in boxes:
+        for single in (False, True):
+            say(f"case periodic fill box {lib.precision} {lo}..{hi} single={single}")
+            mesh = ac.HostMesh(lib, info, data.astype(lib.np_real))
+            with ac.Device(lib, info) as dev:
+                dev.load_mesh(mesh)
+                if single:
+                    ac.check(lib.acDevicePeriodicBoundcondStep(dev.handle, 0, 2, ac.Volume(*lo),
+                                                               ac.Volume(*hi)), "fill step")
+                else:
+                    ac.check(lib.acDevicePeriodicBoundconds(dev.handle, 0, ac.Volume(*lo),
+                                                            ac.Volume(*hi)), "fill")
+                out = ac.HostMesh(lib, info)
+                dev.store_mesh(out)
+            want = data.astype(lib.np_real).copy()
+            sl = (slice(lo[2], hi[2]), slice(lo[1], hi[1]), slice(lo[0], hi[0]))
+            fields = [2] if single else range(ac.NUM_VTXBUF_HANDLES)
+            for q in fields:
+                want[q][sl] = full.astype(lib.np_real)[q][sl]
+            np.testing.assert_array_equal(out.data, want)
+    with ac.Device(lib, info) as dev:
+        assert lib.acDevicePeriodicBoundconds(dev.handle, 0, ac.Volume(3, 0, 0),
+                                              ac.Volume(2, 1, 1)) == ac.AC_FAILURE
+        assert lib.acDevicePeriodicBoundconds(dev.handle, 0, ac.Volume(0, 0, 0),
+                                              ac.Volume(m[0] + 1, m[1], m[2])) == ac.AC_FAILURE
+
+
+def test_load_stencils_from_config(lib):
+    """ref src/core/device.cc:447-473: the stencils stay as loaded until
+    acDeviceLoadStencilsFromConfig evaluates them from the configuration again. A device created
+    with spacing ds, then given 2 ds, integrates with the old stencils until the reload, and
+    afterwards like a device created with 2 ds."""
+    n = 16
+    info1 = lib.config((n, n, n), AC_len=(2 * np.pi,) * 3)
+    info2 = lib.config((n, n, n), AC_len=(4 * np.pi,) * 3)
+    data = c1_initial(lib, info1)
+    dt = 1e-3
+    ref1 = run_device_steps(lib, info1, data, dt, 1)
+    ref2 = run_device_steps(lib, info2, data, dt, 1)
+    assert not np.array_equal(ref1, ref2)
+    for reload in (False, True):
+        mesh = ac.HostMesh(lib, info1, data)
+        with ac.Device(lib, info1) as dev:
+            ac.check(lib.acDeviceLoadMeshInfo(dev.handle, info2), "load info")
+            if reload:
+                ac.check(lib.acDeviceLoadStencilsFromConfig(dev.handle, 0), "stencils")
+            dev.load_mesh(mesh)
+            dev.step(lib.np_real(dt))
+            dev.periodic()
+            out = ac.HostMesh(lib, info1)
+            dev.store_mesh(out)
+        np.testing.assert_array_equal(out.data, ref2 if reload else ref1)
+
+
 def run_device_steps(lib, info, data, dt, steps):
     mesh = ac.HostMesh(lib, info, data)
     with ac.Device(lib, info) as dev:

@@ -65,6 +65,35 @@ def test_default_config_and_derived_params():
     assert info["AC_lupwd"] and info["AC_lentropy"]
 
 
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_single_mesh_dims_setter(prec, tmp_path):
+    """A caller that sets only one of the grid / local extents gets a device of that size: one
+    process holds the whole grid (ref src/core/update_params/update_params.cc:173-177 pushes
+    the grid dims into AC_nlocal in the build without MPI)."""
+    lib = ac.load(prec)
+    info = lib.AcMeshInfo()
+    assert lib.acSetDefaultConfig(ctypes.byref(info)) == 0
+    assert lib.acSetGridMeshDims(64, 48, 40, ctypes.byref(info)) == 0
+    assert info["AC_ngrid"] == (64, 48, 40)
+    assert info["AC_nlocal"] == (64, 48, 40)
+    assert info["AC_mlocal"] == (70, 54, 46)
+    assert info["AC_nlocal_max"] == (67, 51, 43)
+
+    p = tmp_path / "grid.conf"
+    p.write_text("AC_ngrid = {32, 32, 32}\nAC_len = {6.283185307179586, 6.283185307179586, "
+                 "6.283185307179586}\n")
+    info = lib.AcMeshInfo()
+    assert lib.acLoadConfig(str(p).encode(), ctypes.byref(info)) == 0
+    assert info["AC_nlocal"] == (32, 32, 32)
+    assert lib.acSetLocalMeshDims(16, 24, 8, ctypes.byref(info)) == 0
+    assert info["AC_nlocal"] == (16, 24, 8)
+    assert info["AC_ngrid"] == (16, 24, 8)
+    assert info["AC_mlocal"] == (22, 30, 14)
+    # ds follows the extent actually held: len / n
+    np.testing.assert_allclose(info["AC_ds"], (2 * np.pi / 16, 2 * np.pi / 24, 2 * np.pi / 8),
+                               rtol=1e-6)
+
+
 def test_config_loader(tmp_path):
     lib = ac.load("fp64")
     p = tmp_path / "test.conf"
