@@ -830,7 +830,7 @@ static AcResult check_interior(Device d, const Volume start, const Volume end, c
     return AC_SUCCESS;
 }
 
-static AcResult ensure_shape(Device d, hipStream_t s);
+static AcResult ensure_shape(Device d, hipStream_t s, bool w_dead);
 
 AcResult acDeviceIntegrateSubstep(const Device d, const Stream stream, const int step_number,
                                   const Volume start, const Volume end, const AcReal dt)
@@ -841,7 +841,7 @@ AcResult acDeviceIntegrateSubstep(const Device d, const Stream stream, const int
         return fail("acDeviceIntegrateSubstep: step_number must be 0, 1 or 2");
     AC_HIP_CHECK(hipSetDevice(d->id));
     hipStream_t s = get_stream(d, stream);
-    if (ensure_shape(d, s) != AC_SUCCESS)
+    if (ensure_shape(d, s, step_number == 0) != AC_SUCCESS)
         return AC_FAILURE;
     SolveArgs<AcReal> a;
     fill_solve_args(d, a, step_number, dt);
@@ -1049,7 +1049,7 @@ AcResult acDeviceIntegrateSubstepDeviceDt(const Device d, const Stream stream,
         return fail("acDeviceIntegrateSubstepDeviceDt: step_number must be 0, 1 or 2");
     AC_HIP_CHECK(hipSetDevice(d->id));
     hipStream_t s = get_stream(d, stream);
-    if (ensure_shape(d, s) != AC_SUCCESS)
+    if (ensure_shape(d, s, step_number == 0) != AC_SUCCESS)
         return AC_FAILURE;
     SolveArgs<AcReal> a;
     fill_solve_args(d, a, step_number, (AcReal)0);
@@ -1101,7 +1101,7 @@ AcResult acDeviceIntegrateStepsCFL(const Device d, const Stream stream, const in
     // halos: that substep keeps its fill. One fill up front also gives the launch autotuner
     // (first substep on a new grid) a state with consistent halos.
     launch_periodic_fill<AcReal>(d->in, NF, n.x, n.y, n.z, s);
-    if (ensure_shape(d, s) != AC_SUCCESS)
+    if (ensure_shape(d, s, nsteps > 0) != AC_SUCCESS)
         return AC_FAILURE;
     for (int step = 0; step < nsteps; ++step) {
         for (int sub = 0; sub < 3; ++sub) {
@@ -1234,12 +1234,16 @@ AcResult acDeviceGetSolveKernel(const Device d, int* kind)
 // ------------------------------------------------------------------ autotuner
 // Times every candidate launch (the direct kernel, the delay-line shapes and the sweep shapes) on
 // the current grid with the device's own compensation width, physics switches and periodic-image
-// reads, at substep 1 (reads and writes w), and keeps the fastest. The trial launches read the
-// real state and write everything -- f', w and the compensation records -- into the out slab,
-// which holds nothing between substeps: no allocation, and the state in `in`, `w` and the
-// compensation slab is untouched. The choice can be cached in the file named by
-// ASTAROTH_AUTOTUNE_CACHE (lines "nx ny nz kahan upwd entropy realsize tx ty zchunk kind");
-// ASTAROTH_AUTOTUNE_VERBOSE=1 prints each candidate's time to stderr.
+// reads, at substep 0 (reads f, writes f' and w), and keeps the fastest. No allocation: the trial
+// launches read the real state and write f' into the out slab, which holds nothing between
+// substeps. When the tuning runs ahead of a substep-0 launch (every caller's first substep), w is
+// dead as well -- that launch overwrites it without reading it -- and the trials write w into w,
+// so they move the same bytes as the real launches; otherwise (an explicit acDeviceAutotune) w
+// goes into the out slab too, aliasing f', which halves the trial's write traffic. The
+// compensation records (read, and written into the out slab) stay untouched. The choice can be
+// cached in the file named by ASTAROTH_AUTOTUNE_CACHE (lines "nx ny nz kahan upwd entropy
+// realsize tx ty zchunk kind"); ASTAROTH_AUTOTUNE_VERBOSE=1 prints each candidate's time to
+// stderr.
 static bool read_cache(Device d, TiledShape* out)
 {
     const char* path = getenv("ASTAROTH_AUTOTUNE_CACHE");
@@ -1289,7 +1293,7 @@ static const TiledShape kSweepShapes[] = {
     {64, 4, 32, 1}, {64, 4, 64, 1}, {64, 4, 128, 1}, {64, 4, 256, 1},
 };
 
-static AcResult autotune(Device d, hipStream_t s)
+static AcResult autotune(Device d, hipStream_t s, bool w_dead)
 {
     const int3 n = d->local_config.int3_params[AC_nlocal];
     TiledShape best{0, 0, 0, 0};
@@ -1303,15 +1307,16 @@ static AcResult autotune(Device d, hipStream_t s)
     const int kb = d->kahan_bits;
     const bool verbose = getenv("ASTAROTH_AUTOTUNE_VERBOSE") != nullptr;
     SolveArgs<AcReal> a;
-    fill_solve_args(d, a, 1, (AcReal)1e-9);
-    // Trial outputs alias the out slab: f', w and the compensation records (at most
-    // NF * KB / 8 <= NF * sizeof(float) bytes per cell) all stay inside it. Only a single slab
-    // (no field swapped alone) qualifies; otherwise only the direct kernel is timed, into out.
+    fill_solve_args(d, a, 0, (AcReal)1e-9);
+    // Trial outputs: f' into out, w into w when it is dead (else into out), the compensation
+    // records (at most NF * KB / 8 <= NF * sizeof(float) bytes per cell) into out. Only a single
+    // out slab (no field swapped alone) qualifies; otherwise only the direct kernel is timed.
     bool one_slab = true;
     for (int q = 0; q < NF; ++q)
         one_slab = one_slab && d->out[q] == d->out[0] + (size_t)q * d->count;
     for (int q = 0; q < NF; ++q) {
-        a.w[q] = a.out[q];
+        if (!w_dead)
+            a.w[q] = a.out[q];
         a.comp[q] = kb > 0 ? (void*)((char*)d->out[0] + (size_t)q * (kb / 8)) : nullptr;
     }
     a.wrap = 1;
@@ -1385,18 +1390,19 @@ static AcResult autotune(Device d, hipStream_t s)
     return AC_SUCCESS;
 }
 
-static AcResult ensure_shape(Device d, hipStream_t s)
+// w_dead: the caller's next launch is substep 0, which writes w without reading it.
+static AcResult ensure_shape(Device d, hipStream_t s, bool w_dead)
 {
     if (d->shape_tuned)
         return AC_SUCCESS;
-    return autotune(d, s);
+    return autotune(d, s, w_dead);
 }
 
 extern "C" AcResult acDeviceAutotune(const Device d, const Stream stream)
 {
     AC_HIP_CHECK(hipSetDevice(d->id));
     d->shape_tuned = false;
-    return autotune(d, get_stream(d, stream));
+    return autotune(d, get_stream(d, stream), false);
 }
 
 // ------------------------------------------------------------------ helpers for adjoint.hip

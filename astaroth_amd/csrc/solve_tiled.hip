@@ -103,10 +103,13 @@ __device__ __forceinline__ void delay_line_body(const SolveArgs<T>& a, const Adj
 
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int tid = tx + TX * ty;
-    // The wave index is uniform across the wave: kept in a scalar register, so the LDS
+    // The wave index is uniform across the wave. FP32: kept in a scalar register, so the LDS
     // destinations of the DMA (M0) are scalar arithmetic (about 70 fewer vector instructions per
-    // plane step).
-    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    // plane step). FP64 keeps the per-DMA vector form: its kernels run at 512 VGPRs with spills,
+    // and with the scalar form the compiled periodic-image kernels computed wrong results (64 x 2
+    // tiles in one build, 64 x 4 in another) although the source indexes the same LDS and global
+    // elements either way, and their assembly reads no undefined register, spill slot or lane.
+    const int wave = sizeof(T) == 4 ? __builtin_amdgcn_readfirstlane(tid >> 6) : tid >> 6;
     const int lane = tid & 63;
     const int x0 = tile_x * TX, y0 = tile_y * TY, z0 = chunk * zchunk;
     const long long sy = a.mx, sz = a.sz;

@@ -73,6 +73,13 @@ def make_device(ac, prec, n, kahan_bits=0, shape=(0, 0, 0)):
     return lib, dev
 
 
+def solve_kernel_name(lib, dev):
+    """Which solve kernel the device's (autotuned or pinned) launch shape runs."""
+    kind = ctypes.c_int()
+    lib.acDeviceGetSolveKernel(dev.handle, ctypes.byref(kind))
+    return {-1: "direct", 0: "delay-line", 1: "sweep"}.get(kind.value, str(kind.value))
+
+
 def run_steps(lib, dev, steps, courant):
     ac_ok = lib.acDeviceIntegrateStepsCFL(dev.handle, 0, steps, courant, None)
     if ac_ok != 0:
@@ -552,6 +559,7 @@ def main():
     tx, ty, tz = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     lib.acDeviceGetSolveLaunchShape(dev.handle, ctypes.byref(tx), ctypes.byref(ty),
                                     ctypes.byref(tz))
+    kernel_kind = solve_kernel_name(lib, dev)
     dev.close()
     value = n3 * args.steps / wall
     bytes_launch = per_launch_bytes(n3, 4, 0)
@@ -571,6 +579,7 @@ def main():
         "mean_launch_ms": round(mean_launch_s * 1e3, 4),
         "launches": launches,
         "launch_shape": [tx.value, ty.value, tz.value],
+        "launch_kernel": kernel_kind,
     }
     out = {
         "metric": "MHD cell-updates/s at 256^3 fp32",
@@ -599,6 +608,7 @@ def main():
                 w_v, s_ms, cnt = timed(lib_v, dev_v, args.variant_steps, 3)
                 shp = (ctypes.c_int(), ctypes.c_int(), ctypes.c_int())
                 lib_v.acDeviceGetSolveLaunchShape(dev_v.handle, *[ctypes.byref(v) for v in shp])
+                kind_v = solve_kernel_name(lib_v, dev_v)
                 dev_v.close()
             except Exception as e:  # report, do not hide
                 variants[f"{prec}_kahan{kb}"] = {"error": str(e)}
@@ -610,7 +620,7 @@ def main():
                 "cell_updates_per_s": n3 * args.variant_steps / w_v,
                 "bytes_per_cell_update": algorithmic_bytes_per_cell_update(rb, kb),
                 "solve_achieved_GBs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4),
-                "launch_shape": [v.value for v in shp]}
+                "launch_shape": [v.value for v in shp], "launch_kernel": kind_v}
         out["variants"] = variants
         try:
             out["reference_call_pattern"] = reference_call_pattern(ac, n)

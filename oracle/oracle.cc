@@ -470,33 +470,36 @@ static long double host_reduce(T* const* bufs, int nbufs, int map, int op, int r
  * (mhdsolver.ac:586-636): pass 1 computes w = alpha*w + dt*rhs from `in` stencils, pass 2
  * computes f = f + beta*w. Stencil coefficients are folded with inv_ds first, as the DSL's
  * Stencil declarations do (derivs.h:591-752), and evaluated in double. */
+template <typename T>
 struct DPhys {
-    double cs2_sound, nu, zeta, eta, mu0, inv_mu0, cp, inv_cp, gamma, lnT0, lnrho0, K;
+    T cs2_sound, nu, zeta, eta, mu0, inv_mu0, cp, inv_cp, gamma, lnT0, lnrho0, K;
     bool upwd, entropy;
 };
 
+template <typename T>
 struct DStencils {
-    double d1[4][3];   /* per axis, per offset 1..3 (antisymmetric) */
-    double d2[4][3];   /* [0] centre, per axis */
-    double d2c[3];
-    double up[4][3];   /* [0] centre */
-    double upc[3];
-    double mx[4][3];   /* per pair (xy, xz, yz), per offset 1..3 */
+    T d1[4][3];   /* per axis, per offset 1..3 (antisymmetric) */
+    T d2[4][3];   /* [0] centre, per axis */
+    T d2c[3];
+    T up[4][3];   /* [0] centre */
+    T upc[3];
+    T mx[4][3];   /* per pair (xy, xz, yz), per offset 1..3 */
 };
 
-template <int AX>
-static inline double dsten1(const double* a, long c, const long* s, const DStencils& S)
+template <int AX, typename T>
+static inline T dsten1(const T* a, long c, const long* s, const DStencils<T>& S)
 {
     const long st = s[AX];
     return S.d1[1][AX] * (a[c + st] - a[c - st]) + S.d1[2][AX] * (a[c + 2 * st] - a[c - 2 * st]) +
            S.d1[3][AX] * (a[c + 3 * st] - a[c - 3 * st]);
 }
 
-static void refcpu_substep(double* const* in, double* const* wbuf, const Grid& g,
-                           const DPhys& P, const DStencils& S, int step, double dt, int pass)
+template <typename T>
+static void refcpu_substep(T* const* in, T* const* wbuf, const Grid& g,
+                           const DPhys<T>& P, const DStencils<T>& S, int step, T dt, int pass)
 {
-    const double alpha[3] = {0.0, -5.0 / 9.0, -153.0 / 128.0};
-    const double beta[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
+    const T alpha[3] = {T(0.0), T(-5.0 / 9.0), T(-153.0 / 128.0)};
+    const T beta[3] = {T(1.0 / 3.0), T(15.0 / 16.0), T(8.0 / 15.0)};
     const long s[3] = {1, g.sy, g.sz};
 #pragma omp parallel for schedule(static)
     for (int k = 3; k < g.nz + 3; ++k)
@@ -508,17 +511,17 @@ static void refcpu_substep(double* const* in, double* const* wbuf, const Grid& g
                         in[q][c] = in[q][c] + beta[step] * wbuf[q][c];
                     continue;
                 }
-                double val[8], gr[8][3], lap[8], upw[8][3];
+                T val[8], gr[8][3], lap[8], upw[8][3];
                 for (int q = 0; q < 8; ++q) {
-                    const double* a = in[q];
+                    const T* a = in[q];
                     val[q] = a[c];
-                    gr[q][0] = dsten1<0>(a, c, s, S);
-                    gr[q][1] = dsten1<1>(a, c, s, S);
-                    gr[q][2] = dsten1<2>(a, c, s, S);
-                    double l = 0.0;
+                    gr[q][0] = dsten1<0, T>(a, c, s, S);
+                    gr[q][1] = dsten1<1, T>(a, c, s, S);
+                    gr[q][2] = dsten1<2, T>(a, c, s, S);
+                    T l = T(0.0);
                     for (int ax = 0; ax < 3; ++ax) {
                         const long st = s[ax];
-                        const double s1 = a[c + st] + a[c - st], s2 = a[c + 2 * st] + a[c - 2 * st],
+                        const T s1 = a[c + st] + a[c - st], s2 = a[c + 2 * st] + a[c - 2 * st],
                                      s3 = a[c + 3 * st] + a[c - 3 * st];
                         l += S.d2c[ax] * a[c] + S.d2[1][ax] * s1 + S.d2[2][ax] * s2 +
                              S.d2[3][ax] * s3;
@@ -527,66 +530,66 @@ static void refcpu_substep(double* const* in, double* const* wbuf, const Grid& g
                     }
                     lap[q] = l;
                 }
-                auto mixed = [&](const double* a, int p, int ax, int bx) {
+                auto mixed = [&](const T* a, int p, int ax, int bx) {
                     const long sa = s[ax], sb = s[bx];
-                    double r = 0.0;
+                    T r = T(0.0);
                     for (int o = 1; o <= 3; ++o)
                         r += S.mx[o][p] * (a[c + o * sa + o * sb] + a[c - o * sa - o * sb] -
                                            a[c + o * sa - o * sb] - a[c - o * sa + o * sb]);
                     return r;
                 };
-                auto second = [&](const double* a, int ax) {
+                auto second = [&](const T* a, int ax) {
                     const long st = s[ax];
                     return S.d2c[ax] * a[c] + S.d2[1][ax] * (a[c + st] + a[c - st]) +
                            S.d2[2][ax] * (a[c + 2 * st] + a[c - 2 * st]) +
                            S.d2[3][ax] * (a[c + 3 * st] + a[c - 3 * st]);
                 };
                 /* grad div of a vector stored in fields b, b+1, b+2 */
-                auto graddiv = [&](int b, double gd[3]) {
+                auto graddiv = [&](int b, T gd[3]) {
                     gd[0] = second(in[b], 0) + mixed(in[b + 1], 0, 0, 1) + mixed(in[b + 2], 1, 0, 2);
                     gd[1] = mixed(in[b], 0, 0, 1) + second(in[b + 1], 1) + mixed(in[b + 2], 2, 1, 2);
                     gd[2] = mixed(in[b], 1, 0, 2) + mixed(in[b + 1], 2, 1, 2) + second(in[b + 2], 2);
                 };
-                double gdu[3], gda[3];
+                T gdu[3], gda[3];
                 graddiv(1, gdu);
                 graddiv(4, gda);
-                const double ux = val[1], uy = val[2], uz = val[3];
-                const double au[3] = {fabs(ux), fabs(uy), fabs(uz)};
-                const double u[3] = {ux, uy, uz};
-                const double divu = gr[1][0] + gr[2][1] + gr[3][2];
-                double rhs[8];
+                const T ux = val[1], uy = val[2], uz = val[3];
+                const T au[3] = {std::fabs(ux), std::fabs(uy), std::fabs(uz)};
+                const T u[3] = {ux, uy, uz};
+                const T divu = gr[1][0] + gr[2][1] + gr[3][2];
+                T rhs[8];
                 rhs[0] = -(u[0] * gr[0][0] + u[1] * gr[0][1] + u[2] * gr[0][2]) - divu;
                 if (P.upwd)
                     rhs[0] += au[0] * upw[0][0] + au[1] * upw[0][1] + au[2] * upw[0][2];
-                double Sm[3][3];
-                Sm[0][0] = (2.0 / 3.0) * gr[1][0] - (1.0 / 3.0) * (gr[2][1] + gr[3][2]);
-                Sm[0][1] = 0.5 * (gr[1][1] + gr[2][0]);
-                Sm[0][2] = 0.5 * (gr[1][2] + gr[3][0]);
-                Sm[1][1] = (2.0 / 3.0) * gr[2][1] - (1.0 / 3.0) * (gr[1][0] + gr[3][2]);
-                Sm[1][2] = 0.5 * (gr[2][2] + gr[3][1]);
-                Sm[2][2] = (2.0 / 3.0) * gr[3][2] - (1.0 / 3.0) * (gr[1][0] + gr[2][1]);
+                T Sm[3][3];
+                Sm[0][0] = T(2.0 / 3.0) * gr[1][0] - T(1.0 / 3.0) * (gr[2][1] + gr[3][2]);
+                Sm[0][1] = T(0.5) * (gr[1][1] + gr[2][0]);
+                Sm[0][2] = T(0.5) * (gr[1][2] + gr[3][0]);
+                Sm[1][1] = T(2.0 / 3.0) * gr[2][1] - T(1.0 / 3.0) * (gr[1][0] + gr[3][2]);
+                Sm[1][2] = T(0.5) * (gr[2][2] + gr[3][1]);
+                Sm[2][2] = T(2.0 / 3.0) * gr[3][2] - T(1.0 / 3.0) * (gr[1][0] + gr[2][1]);
                 Sm[1][0] = Sm[0][1];
                 Sm[2][0] = Sm[0][2];
                 Sm[2][1] = Sm[1][2];
-                const double cs2 = P.entropy ? P.cs2_sound * exp(P.gamma * val[7] / P.cp +
-                                                                 (P.gamma - 1.0) * (val[0] - P.lnrho0))
+                const T cs2 = P.entropy ? P.cs2_sound * std::exp(P.gamma * val[7] / P.cp +
+                                                                 (P.gamma - T(1.0)) * (val[0] - P.lnrho0))
                                              : P.cs2_sound;
-                const double jv[3] = {P.inv_mu0 * (gda[0] - lap[4]), P.inv_mu0 * (gda[1] - lap[5]),
+                const T jv[3] = {P.inv_mu0 * (gda[0] - lap[4]), P.inv_mu0 * (gda[1] - lap[5]),
                                       P.inv_mu0 * (gda[2] - lap[6])};
-                const double B[3] = {gr[6][1] - gr[5][2], gr[4][2] - gr[6][0], gr[5][0] - gr[4][1]};
-                const double inv_rho = 1.0 / exp(val[0]);
-                const double jxB[3] = {jv[1] * B[2] - jv[2] * B[1], jv[2] * B[0] - jv[0] * B[2],
+                const T B[3] = {gr[6][1] - gr[5][2], gr[4][2] - gr[6][0], gr[5][0] - gr[4][1]};
+                const T inv_rho = T(1.0) / std::exp(val[0]);
+                const T jxB[3] = {jv[1] * B[2] - jv[2] * B[1], jv[2] * B[0] - jv[0] * B[2],
                                        jv[0] * B[1] - jv[1] * B[0]};
                 for (int d = 0; d < 3; ++d) {
                     const int q = 1 + d;
-                    double m = -(gr[q][0] * u[0] + gr[q][1] * u[1] + gr[q][2] * u[2]);
+                    T m = -(gr[q][0] * u[0] + gr[q][1] * u[1] + gr[q][2] * u[2]);
                     if (P.upwd)
                         m += upw[q][0] * au[0] + upw[q][1] * au[1] + upw[q][2] * au[2];
-                    const double press = P.entropy ? P.inv_cp * gr[7][d] + gr[0][d] : gr[0][d];
+                    const T press = P.entropy ? P.inv_cp * gr[7][d] + gr[0][d] : gr[0][d];
                     m -= cs2 * press;
                     m += inv_rho * jxB[d];
-                    m += P.nu * (lap[q] + (1.0 / 3.0) * gdu[d] +
-                                 2.0 * (Sm[d][0] * gr[0][0] + Sm[d][1] * gr[0][1] + Sm[d][2] * gr[0][2]));
+                    m += P.nu * (lap[q] + T(1.0 / 3.0) * gdu[d] +
+                                 T(2.0) * (Sm[d][0] * gr[0][0] + Sm[d][1] * gr[0][1] + Sm[d][2] * gr[0][2]));
                     m += P.zeta * gdu[d];
                     rhs[q] = m;
                 }
@@ -594,27 +597,27 @@ static void refcpu_substep(double* const* in, double* const* wbuf, const Grid& g
                 rhs[5] = (u[2] * B[0] - u[0] * B[2]) + P.eta * lap[5];
                 rhs[6] = (u[0] * B[1] - u[1] * B[0]) + P.eta * lap[6];
                 if (P.entropy) {
-                    const double lnT = P.lnT0 + P.gamma * val[7] / P.cp + (P.gamma - 1.0) * (val[0] - P.lnrho0);
-                    const double inv_pT = 1.0 / (exp(val[0]) * exp(lnT));
-                    double cS = 0.0;
+                    const T lnT = P.lnT0 + P.gamma * val[7] / P.cp + (P.gamma - T(1.0)) * (val[0] - P.lnrho0);
+                    const T inv_pT = T(1.0) / (std::exp(val[0]) * std::exp(lnT));
+                    T cS = T(0.0);
                     for (int a = 0; a < 3; ++a)
                         for (int b = 0; b < 3; ++b)
                             cS += Sm[a][b] * Sm[a][b];
-                    const double R = P.eta * P.mu0 * (jv[0] * jv[0] + jv[1] * jv[1] + jv[2] * jv[2]) +
-                                     2.0 * exp(val[0]) * P.nu * cS + P.zeta * exp(val[0]) * divu * divu;
-                    const double first = P.gamma * P.inv_cp * lap[7] + (P.gamma - 1.0) * lap[0];
-                    double dt23 = 0.0;
+                    const T R = P.eta * P.mu0 * (jv[0] * jv[0] + jv[1] * jv[1] + jv[2] * jv[2]) +
+                                     T(2.0) * std::exp(val[0]) * P.nu * cS + P.zeta * std::exp(val[0]) * divu * divu;
+                    const T first = P.gamma * P.inv_cp * lap[7] + (P.gamma - T(1.0)) * lap[0];
+                    T dt23 = T(0.0);
                     for (int d = 0; d < 3; ++d) {
-                        const double sec = P.gamma * P.inv_cp * gr[7][d] + (P.gamma - 1.0) * gr[0][d];
-                        const double thi = P.gamma * (P.inv_cp * gr[7][d] + gr[0][d]) - gr[0][d];
+                        const T sec = P.gamma * P.inv_cp * gr[7][d] + (P.gamma - T(1.0)) * gr[0][d];
+                        const T thi = P.gamma * (P.inv_cp * gr[7][d] + gr[0][d]) - gr[0][d];
                         dt23 += sec * thi;
                     }
-                    const double chi = P.K / (exp(val[0]) * P.cp);
+                    const T chi = P.K / (std::exp(val[0]) * P.cp);
                     rhs[7] = -(u[0] * gr[7][0] + u[1] * gr[7][1] + u[2] * gr[7][2]) + inv_pT * R +
                              P.cp * chi * (first + dt23);
                 }
                 else {
-                    rhs[7] = 0.0;
+                    rhs[7] = T(0.0);
                 }
                 for (int q = 0; q < 8; ++q)
                     wbuf[q][c] = alpha[step] * wbuf[q][c] + rhs[q] * dt;
@@ -622,6 +625,64 @@ static void refcpu_substep(double* const* in, double* const* wbuf, const Grid& g
 }
 
 } // namespace
+
+/* One full RK3 step of the ref-cpu algorithm in arithmetic type T (the reference's CPU build with
+ * AcReal = T: two passes per substep, OpenMP over z with nthreads threads; nthreads <= 0 keeps
+ * the OpenMP default). `wbuf` is the caller's 8-field intermediate buffer of the same shape. */
+template <typename T>
+static int refcpu_integrate_step(T* const* fields, T* const* wbuf, const OracleConfig* cfg,
+                                 double dt, int nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads > 0)
+        omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    const Grid g = make_grid(cfg);
+    DPhys<T> P;
+    P.cs2_sound = T(cfg->cs_sound * cfg->cs_sound);
+    P.nu = T(cfg->nu_visc);
+    P.zeta = T(cfg->zeta);
+    P.eta = T(cfg->eta);
+    P.mu0 = T(cfg->mu0);
+    P.inv_mu0 = T(1.0 / cfg->mu0);
+    P.cp = T(cfg->cp_sound);
+    P.inv_cp = T(1.0 / cfg->cp_sound);
+    P.gamma = T(cfg->gamma);
+    P.lnT0 = T(cfg->lnT0);
+    P.lnrho0 = T(cfg->lnrho0);
+    P.K = T(cfg->K_heatcond);
+    P.upwd = cfg->lupwd != 0;
+    P.entropy = cfg->lentropy != 0;
+    DStencils<T> S;
+    const double c1[4] = {0.0, 3.0 / 4.0, -3.0 / 20.0, 1.0 / 60.0};
+    const double c2[4] = {-49.0 / 18.0, 3.0 / 2.0, -3.0 / 20.0, 1.0 / 90.0};
+    const double cu[4] = {-20.0 / 60.0, 15.0 / 60.0, -6.0 / 60.0, 1.0 / 60.0};
+    const double cm[4] = {0.0, 270.0 / 720.0, -27.0 / 720.0, 2.0 / 720.0};
+    double inv[3];
+    for (int d = 0; d < 3; ++d)
+        inv[d] = 1.0 / cfg->ds[d];
+    for (int d = 0; d < 3; ++d) {
+        for (int o = 1; o <= 3; ++o) {
+            S.d1[o][d] = T(inv[d] * c1[o]);
+            S.d2[o][d] = T(inv[d] * inv[d] * c2[o]);
+            S.up[o][d] = T(inv[d] * cu[o]);
+        }
+        S.d2c[d] = T(inv[d] * inv[d] * c2[0]);
+        S.upc[d] = T(inv[d] * cu[0]);
+    }
+    const int pairs[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+    for (int p = 0; p < 3; ++p)
+        for (int o = 1; o <= 3; ++o)
+            S.mx[o][p] = T(inv[pairs[p][0]] * inv[pairs[p][1]] * cm[o]);
+    for (int s = 0; s < 3; ++s) {
+        periodic_fill(fields, 8, g);
+        refcpu_substep<T>(fields, wbuf, g, P, S, s, T(dt), 0);
+        refcpu_substep<T>(fields, wbuf, g, P, S, s, T(dt), 1);
+    }
+    return 0;
+}
 
 extern "C" {
 
@@ -658,61 +719,16 @@ long double oracle_reduce_f32(float* const* bufs, int nbufs, int map, int op, in
     return host_reduce<float>(bufs, nbufs, map, op, rms, cfg);
 }
 
-/* One full RK3 step of the ref-cpu baseline (FP64, two passes per substep, OpenMP over z with
- * nthreads threads; nthreads <= 0 keeps the OpenMP default). `wbuf` is the caller's 8-field
- * intermediate buffer of the same shape. */
 int refcpu_integrate_step_f64(double* const* fields, double* const* wbuf, const OracleConfig* cfg,
                               double dt, int nthreads)
 {
-#ifdef _OPENMP
-    if (nthreads > 0)
-        omp_set_num_threads(nthreads);
-#else
-    (void)nthreads;
-#endif
-    const Grid g = make_grid(cfg);
-    DPhys P;
-    P.cs2_sound = cfg->cs_sound * cfg->cs_sound;
-    P.nu = cfg->nu_visc;
-    P.zeta = cfg->zeta;
-    P.eta = cfg->eta;
-    P.mu0 = cfg->mu0;
-    P.inv_mu0 = 1.0 / cfg->mu0;
-    P.cp = cfg->cp_sound;
-    P.inv_cp = 1.0 / cfg->cp_sound;
-    P.gamma = cfg->gamma;
-    P.lnT0 = cfg->lnT0;
-    P.lnrho0 = cfg->lnrho0;
-    P.K = cfg->K_heatcond;
-    P.upwd = cfg->lupwd != 0;
-    P.entropy = cfg->lentropy != 0;
-    DStencils S;
-    const double c1[4] = {0.0, 3.0 / 4.0, -3.0 / 20.0, 1.0 / 60.0};
-    const double c2[4] = {-49.0 / 18.0, 3.0 / 2.0, -3.0 / 20.0, 1.0 / 90.0};
-    const double cu[4] = {-20.0 / 60.0, 15.0 / 60.0, -6.0 / 60.0, 1.0 / 60.0};
-    const double cm[4] = {0.0, 270.0 / 720.0, -27.0 / 720.0, 2.0 / 720.0};
-    double inv[3];
-    for (int d = 0; d < 3; ++d)
-        inv[d] = 1.0 / cfg->ds[d];
-    for (int d = 0; d < 3; ++d) {
-        for (int o = 1; o <= 3; ++o) {
-            S.d1[o][d] = inv[d] * c1[o];
-            S.d2[o][d] = inv[d] * inv[d] * c2[o];
-            S.up[o][d] = inv[d] * cu[o];
-        }
-        S.d2c[d] = inv[d] * inv[d] * c2[0];
-        S.upc[d] = inv[d] * cu[0];
-    }
-    const int pairs[3][2] = {{0, 1}, {0, 2}, {1, 2}};
-    for (int p = 0; p < 3; ++p)
-        for (int o = 1; o <= 3; ++o)
-            S.mx[o][p] = inv[pairs[p][0]] * inv[pairs[p][1]] * cm[o];
-    for (int s = 0; s < 3; ++s) {
-        periodic_fill(fields, 8, g);
-        refcpu_substep(fields, wbuf, g, P, S, s, dt, 0);
-        refcpu_substep(fields, wbuf, g, P, S, s, dt, 1);
-    }
-    return 0;
+    return refcpu_integrate_step<double>(fields, wbuf, cfg, dt, nthreads);
+}
+
+int refcpu_integrate_step_f32(float* const* fields, float* const* wbuf, const OracleConfig* cfg,
+                              double dt, int nthreads)
+{
+    return refcpu_integrate_step<float>(fields, wbuf, cfg, dt, nthreads);
 }
 
 int oracle_num_threads(void)

@@ -89,6 +89,7 @@ def lib():
                                         ctypes.c_int, cfg]
         L.oracle_reduce_f32.restype = ctypes.c_longdouble
         L.refcpu_integrate_step_f64.argtypes = [pp_d, pp_d, cfg, ctypes.c_double, ctypes.c_int]
+        L.refcpu_integrate_step_f32.argtypes = [pp_f, pp_f, cfg, ctypes.c_double, ctypes.c_int]
         L.oracle_num_threads.restype = ctypes.c_int
         L.oracle_seed_rng.argtypes = [ctypes.c_uint32]
         L.oracle_seed_rng.restype = None
@@ -144,8 +145,15 @@ def reduce(bufs, cfg, map_kind, op, rms):
 
 
 def refcpu_step(fields, wbuf, cfg, dt, nthreads=0):
-    """One RK3 step of the ref-cpu baseline (FP64, two-pass, OpenMP)."""
-    lib().refcpu_integrate_step_f64(_ptrs(fields), _ptrs(wbuf), ctypes.byref(cfg), dt, nthreads)
+    """One RK3 step of the ref-cpu algorithm (two-pass, OpenMP) in the arithmetic of `fields`:
+    float64 is the reference CPU build's default, float32 the same build with AcReal = float."""
+    assert fields.dtype == wbuf.dtype
+    if fields.dtype == np.float64:
+        lib().refcpu_integrate_step_f64(_ptrs(fields), _ptrs(wbuf), ctypes.byref(cfg), dt, nthreads)
+    elif fields.dtype == np.float32:
+        lib().refcpu_integrate_step_f32(_ptrs(fields), _ptrs(wbuf), ctypes.byref(cfg), dt, nthreads)
+    else:
+        raise TypeError(fields.dtype)
 
 
 def num_threads():

@@ -210,11 +210,25 @@ def test_scale_adjoint_384_revolve(require_gpu):
                     assert rel < 1e-6
             finally:
                 runner.close()
+    f0_dir_shape = np.zeros(f0_64.shape, dtype=np.float64)
     del f0_64, tgt_64, f0, tgt
     a, b = grads[("fp64", 1)], grads[("fp64", 2)]
     assert torch.equal(a[0], b[0]) and a[1] == b[1] and a[2] == b[2]
     g64 = grads[("fp64", 2)][0]
     g32 = grads[("fp32", 2)][0].double()
-    rel = float((g64 - g32).abs().max() / g64.abs().max())
-    say(f"  max |g32 - g64| / max |g64| = {rel:.3g}")
-    assert rel < 1e-4
+    d = g64 - g32
+    rel_max = float(d.abs().max() / g64.abs().max())
+    rel_l2 = float(d.norm() / g64.norm())
+    v = torch.from_numpy(TA.random_direction(f0_dir_shape, 33)).to(g64.device)
+    rel_dir = abs(float((d * v).sum())) / abs(float((g64 * v).sum()))
+    idx = np.unravel_index(int(d.abs().argmax()), tuple(d.shape))
+    say(f"  |g32 - g64| / |g64|: max-norm {rel_max:.3g} at {idx}, L2 {rel_l2:.3g}, "
+        f"along a random direction {rel_dir:.3g}")
+    # The FP32 rounding of the second-derivative stencils is of order eps |f| / dx^2, 576 times
+    # its size at the 16^3 check of test_gpu_adjoint (max-norm 1e-4 there): the pointwise bound
+    # is 2e-3 here (measured 5.8e-4). The L2 and directional differences average that rounding
+    # out (measured 2.5e-5 and 2.2e-5); they are held to the 1e-4 of the FP32 gradient in the
+    # differentiation form.
+    assert rel_max < 2e-3
+    assert rel_l2 < 1e-4
+    assert rel_dir < 1e-4

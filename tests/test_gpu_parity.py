@@ -179,31 +179,52 @@ def oracle_steps(info, data, dt, steps):
     return model
 
 
+def refcpu_steps(info, data, dt, steps):
+    """The reference CPU build's algorithm (two passes per substep, in the build's precision) on
+    the same data: an independent implementation whose operation order differs from the model's."""
+    cand = data.copy()
+    w = np.zeros_like(cand)
+    cfg = O.config_from_info(info)
+    for _ in range(steps):
+        O.refcpu_step(cand, w, cfg, dt, 8)
+    O.periodic_fill(cand, cfg)
+    return cand
+
+
 @pytest.mark.parametrize("n,steps,max_ulp,entropy", [(8, 100, 34, False), (32, 100, 5, False),
                                                      (32, 100, 8, True)])
 def test_devicetest_integration(lib, n, steps, max_ulp, entropy):
     """ref samples/devicetest/main.cc:84-115 (and mpitest at 8^3 / 34 ulp). The reference runs
     these with its shipped DSL flags (ref mhdsolver.ac:99,101: LENTROPY 0, LUPWD 1); the
-    entropy-on variant (all 8 fields evolved) is held to 8 ulp."""
+    entropy-on variant (all 8 fields evolved) is held to 8 ulp.
+
+    Acceptance per field: the reference's rule (ref src/utils/verification.cc:94-107: the error
+    at the point of largest absolute error below max_ulp ulp of the model value there, or below
+    min|model| eps). The reference CPU build's own algorithm fails that rule on these cases too
+    (FP64 32^3: u_x 7 ulp; FP32 32^3: u_y 7 ulp): with dt = FLT_EPSILON every substep rounds the
+    state update, and any operation order other than the model's flips some of those roundings.
+    A field that fails the rule therefore passes only if its largest absolute error is at most
+    twice that of the reference CPU build's algorithm on the same case and field; both verdicts
+    are printed with the cell and model value."""
     say(f"case devicetest integration {lib.precision} n={n} steps={steps} entropy={entropy}")
     info = lib.config((n, n, n), AC_lentropy=entropy)
     data = random_mesh(lib, info, 321654987)
     cand = run_device_steps(lib, info, data, FLT_EPSILON, steps)
     model = oracle_steps(info, data, FLT_EPSILON, steps)
+    cpu = refcpu_steps(info, data, FLT_EPSILON, steps)
     full = (slice(None),) * 3
     bad = []
     bits = BITS[lib.precision]
     for q in range(8):
         e = O.verify_field(model[q], cand[q], full, max_ulp, bits)
-        # The reference's metric reads the ulp error at the point of the largest absolute error,
-        # which over-weights small values; the same error is also accepted when it is within
-        # 4 ulp of the field's largest magnitude (the random-data dynamics reach |u| ~ 25).
-        field_ulp = 2.0 ** (np.floor(np.log2(e["max_mag"])) - (bits - 1)) if e["max_mag"] > 0 else 0
-        scale_ok = e["abs"] <= 4 * field_ulp
-        say(f"  {ac.FIELD_NAMES[q]}: ulp={e['ulp']:.3g} abs={e['abs']:.3g} "
-            f"field-ulps={e['abs'] / field_ulp if field_ulp else 0:.3g} ok={e['ok'] or scale_ok}")
-        if not (e["ok"] or scale_ok):
-            bad.append((q, e))
+        r = O.verify_field(model[q], cpu[q], full, max_ulp, bits)
+        cal_ok = e["abs"] <= 2.0 * r["abs"]
+        say(f"  {ac.FIELD_NAMES[q]}: ulp={e['ulp']:.3g} abs={e['abs']:.3g} at {e['where']} "
+            f"(model {model[q][e['where']]:.6g}) reference rule ok={e['ok']}; reference CPU "
+            f"algorithm ulp={r['ulp']:.3g} abs={r['abs']:.3g} ok={r['ok']}; within 2x its "
+            f"abs={cal_ok}")
+        if not (e["ok"] or cal_ok):
+            bad.append((q, e, r))
     assert not bad, bad
 
 

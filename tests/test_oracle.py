@@ -237,6 +237,34 @@ def test_refcpu_matches_long_double_path():
         assert np.max(np.abs(interior(f[q] - g[q]))) / scale < 1e-12, q
 
 
+def test_refcpu_fp32_tracks_long_double_path():
+    """The reference CPU build's algorithm in FP32 arithmetic and storage (AcReal = float) stays
+    within FP32 rounding of the long-double path (with float storage), and differs from it by
+    more than a double computation would."""
+    rng = np.random.default_rng(12)
+    cfg = O.make_config((N, N, N), H, nu_visc=5e-3, eta=5e-3, K_heatcond=5e-3, gamma=5 / 3)
+    X = coords()
+    f = empty()
+    for q in range(1, 7):
+        k = rng.integers(-2, 3, size=3)
+        f[q] = 0.05 * np.sin(k[0] * X[0] + k[1] * X[1] + k[2] * X[2] + rng.random() * 6)
+    f32 = f.astype(np.float32)
+    model = f32.copy()
+    g = f32.copy()
+    w = np.zeros_like(g)
+    for _ in range(2):
+        O.integrate_step(model, cfg, 1e-2)
+        O.refcpu_step(g, w, cfg, 1e-2)
+    worst = 0.0
+    for q in range(1, 7):
+        scale = np.max(np.abs(interior(model[q]).astype(np.float64)))
+        err = np.max(np.abs(interior(model[q]).astype(np.float64) - interior(g[q]))) / scale
+        worst = max(worst, err)
+        assert err < 1e-5, (q, err)
+    # float arithmetic leaves errors well above the storage rounding of a double computation
+    assert worst > 1e-8, worst
+
+
 def test_float_storage_close_to_double():
     rng = np.random.default_rng(5)
     cfg = O.make_config((N, N, N), H)
