@@ -102,6 +102,7 @@ EXPORTED = [
     "acDeviceWriteSnapshot", "acDeviceReadSnapshot", "acHostAppendSnapshotInfo",
     "acHostReadSnapshotInfo", "seed_rng", "random_uniform_real_01", "generateHelicalForcingParams",
     "loadForcingParamsToMeshInfo", "printForcingParams", "acDeviceLoadForcingParams",
+    "acReduceNoAlloc",
 ]
 
 
@@ -312,6 +313,8 @@ class AstarothLib:
             "acHostReadSnapshotInfo": [ctypes.c_char_p, P(i), P(i), P(ctypes.c_double)],
             "loadForcingParamsToMeshInfo": [self.ForcingParams, P(Info)],
             "acDeviceLoadForcingParams": [dev, i, self.ForcingParams],
+            "acReduceNoAlloc": [AcReduction, P(ctypes.c_void_p), i, i, i, i, i, ctypes.c_void_p,
+                                ctypes.c_void_p],
         }
         sig["acHostAppendTimeseries"] = [ctypes.c_char_p, i, ctypes.c_double, ctypes.c_double,
                                          ctypes.POINTER(AcDiagnostics), ctypes.POINTER(ctypes.c_int)]

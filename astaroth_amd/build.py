@@ -16,7 +16,7 @@ LIBDIR = os.path.join(HERE, "lib")
 OBJDIR = os.path.join(HERE, "_build")
 
 SOURCES = ["device.hip", "solve_direct.hip", "solve_tiled.hip", "solve_ring.hip", "solve_sweep.hip",
-           "aux_kernels.hip", "ic.hip", "host.cc", "adjoint.hip", "forcing.cc"]
+           "aux_kernels.hip", "ic.hip", "host.cc", "adjoint.hip", "forcing.cc", "noalloc.hip"]
 ARCH = os.environ.get("ASTAROTH_OFFLOAD_ARCH", "gfx950")
 
 HIP_FLAGS = [

@@ -405,6 +405,23 @@ AcResult acDeviceReduceVecScal(const Device device, const Stream stream,
                                const VertexBufferHandle vtxbuf1, const VertexBufferHandle vtxbuf2,
                                const VertexBufferHandle vtxbuf3, AcReal* result);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the reduction of
+ * acDeviceReduceScal (num_vtxbufs = 1), acDeviceReduceVec (3) or acDeviceReduceVecScal (4), with
+ * (post_process != 0) or without the AC_RMS post-processing, over caller-owned device buffers of
+ * (nx + 6) (ny + 6) (nz + 6) values each (the halo'ed layout of the device's vertex buffers,
+ * x fastest; only the interior is reduced), enqueued on the HIP stream `stream` (0: the null
+ * stream). The AcReal result is written to the device address `result`; nothing synchronises
+ * the host. Its partial results live in a fixed device array of 1025 doubles (8 KiB per GPU,
+ * held until the process ends), which bounds the first stage at 1024 workgroups for every grid
+ * size; no allocation is made. The same kernels as the allocating entry points: the same value
+ * bit for bit. Calls that overlap on one GPU share that array and race. Returns AC_FAILURE
+ * (nothing launched) for null pointers, non-positive extents or a reduction type undefined for
+ * the number of buffers. */
+AcResult acReduceNoAlloc(const AcReduction reduction, const AcReal* const* vtxbufs,
+                         const int num_vtxbufs, const int nx, const int ny, const int nz,
+                         const int post_process, void* stream, AcReal* result);
+
 /* ---- Extensions with no one-call counterpart in the reference ---- */
 
 /* Timestep formula of the device-resident dt below.
