@@ -391,6 +391,38 @@ def form_scale(ac, n=1024, steps=50, kb=16, seed=1024):
             "solve_achieved_GBs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4)}
 
 
+def form_reductions(ac, n=256, reps=50, seed=2024):
+    """The on-device reductions at 256^3 FP32 (the vrms / max|u| / per-field min-max path):
+    acDeviceReduceScal (RTYPE_MAX, RTYPE_RMS), acDeviceReduceVec (RTYPE_MAX), acDeviceReduceVecScal
+    (RTYPE_ALFVEN_MAX) and the one-pass acDeviceReduceDiagnostics (every field plus |u|, |curl A|,
+    v_A). Host-timed per call (each call synchronises: it returns the value to the host); the
+    algorithmic bytes are the interior values read (4 B each)."""
+    lib, dev = make_device(ac, "fp32", n)
+    try:
+        dev.periodic()
+        dev.sync()
+        n3 = n ** 3
+        calls = {
+            "scal_max": (lambda: dev.reduce_scal("RTYPE_MAX", 1), 1),
+            "scal_rms": (lambda: dev.reduce_scal("RTYPE_RMS", 1), 1),
+            "vec_max": (lambda: dev.reduce_vec("RTYPE_MAX", 1, 2, 3), 3),
+            "vecscal_alfven_max": (lambda: dev.reduce_vec_scal("RTYPE_ALFVEN_MAX", 4, 5, 6, 0), 4),
+            "diagnostics": (lambda: dev.diagnostics(), NF),
+        }
+        out = {"form": "reductions", "grid": [n, n, n], "dtype": "f32", "reps": reps}
+        for name, (fn, nbuf) in calls.items():
+            fn()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                fn()
+            t = (time.perf_counter() - t0) / reps
+            out[name] = {"us_per_call": round(t * 1e6, 1),
+                         "GBs_algorithmic": round(nbuf * n3 * 4 / t / 1e9, 1)}
+        return out
+    finally:
+        dev.close()
+
+
 def form_scale_adjoint(ac, n=384, steps=500, snaps=24, seed=384):
     """Adjoint of a 384^3 FP32 run of 500 fixed-dt steps with revolve checkpointing (snapshot
     slots sized to the HBM budget), loss 0.5 |u(T) - u_target|^2 with u_target the initial u of a
@@ -528,7 +560,7 @@ def main():
     ap.add_argument("--variant-steps", type=int, default=30)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--form", choices=["precision", "precision_plain", "scale", "scale_adjoint",
-                                       "differentiation"],
+                                       "differentiation", "reductions"],
                     help="run one measurement form instead of the headline benchmark")
     ap.add_argument("--shape", type=int, nargs=3, default=(0, 0, 0),
                     help="pin the solve launch shape TX TY ZC (0 0 0 = autotune)")
@@ -543,7 +575,7 @@ def main():
 
     from astaroth_amd import astaroth as ac
     if args.form:
-        fn = {"precision": form_precision, "scale": form_scale,
+        fn = {"precision": form_precision, "scale": form_scale, "reductions": form_reductions,
               "scale_adjoint": form_scale_adjoint, "differentiation": form_differentiation,
               # the plain-method workload (256^3, 16 modes |k| <= 4, amplitude 0.05, 100 steps,
               # dt = 0.2 dx / (cs + max|u|)) in the three precisions
