@@ -76,10 +76,12 @@ __device__ __forceinline__ void delay_line_body(const SolveArgs<T>& a, const Adj
     // LATE: the output of plane p - 3 (with its stores) runs after the end-of-step barrier, and
     // the DMA runs two planes ahead, issued right after that barrier into the buffer just
     // released. The barrier's wait then finds the previous step's stores and the next plane's
-    // DMA long done, instead of waiting for the stores just issued. Used by the adjoint's local
-    // stage (55 stores per cell, one workgroup per CU) in FP32; in FP64 the longer-lived
-    // registers would spill.
-    constexpr bool LATE = ADJ && sizeof(T) == 4;
+    // DMA long done, instead of waiting for the stores just issued (vmcnt cannot tell loads from
+    // stores on this target, so the wait before the barrier covers both). Used in FP32 by the
+    // adjoint's local stage (55 stores per cell) and by the forward kernels up to 16-bit
+    // compensation (plain FP32 0.598 -> 0.584 ms per launch at 256^3, 16-bit 0.68 -> 0.66 ms);
+    // the 32-bit compensated and FP64 kernels would spill the longer-lived registers.
+    constexpr bool LATE = sizeof(T) == 4 && (ADJ || KB < 32);
     // Skip the in-plane stage (3) on the planes that are not output. Measured: FP32 plain, 8- and
     // 16-bit compensated kernels 0-5% faster; the 32-bit compensated and FP64 kernels 1.5-2%
     // slower (different register allocation), so they keep the stage on every plane.
