@@ -5,11 +5,12 @@
 // Adjoint:  mu = lw' + beta_s lf',  lw = alpha_s mu,  lf = lf' + dt J(f)^T mu,
 //           lnu += dt <mu, dR/dnu>,  leta += dt <mu, dR/deta>.
 // J^T mu is computed in three stages: (A) per cell, the adjoint of every derivative quantity
-// the right-hand side reads (55 fields, "derivative adjoints"); (B) their periodic halo fill,
-// which is the transpose of the forward halo copy; (C) per output field, the transposed
-// stencils applied to the derivative adjoints by a z-marching kernel (first derivatives are
-// antisymmetric, so their transpose is the negative; second, upwind and mixed ones are
-// symmetric).
+// the right-hand side reads at a neighbour (NG = 47 fields, "derivative adjoints"), together
+// with the pointwise parts: lw = alpha mu and lf += dt * (the adjoint of the cell's own value);
+// (B) the periodic halo fill of the derivative adjoints, which is the transpose of the forward
+// halo copy; (C) per output field, the transposed stencils applied to the derivative adjoints
+// by a z-marching kernel, added to lf (first derivatives are antisymmetric, so their transpose
+// is the negative; second, upwind and mixed ones are symmetric).
 #include <cstdio>
 #include <cstring>
 
@@ -183,7 +184,15 @@ __global__ void __launch_bounds__(256) adjoint_local_kernel(AdjArgs<T> A)
         peta = (double)a.dt * (double)aeta;
         T* g = A.g;
         const long long n = A.count;
-        put_derivative_adjoints(ad, aup, [&](int k, T v) { g[(long long)k * n + c] = v; });
+        put_derivative_adjoints<UPWD>(ad, aup, [&](int k, T v) { g[(long long)k * n + c] = v; });
+#pragma unroll
+        for (int q = 0; q < NF; ++q) {
+            const long long o = (long long)q * n + c;
+            const T lfo = A.lf[o];
+            A.lw[o] = a.alpha * mu[q];
+            if (q < 4 || (q == 7 && ENTROPY))
+                A.lf[o] = lfo + a.dt * value_adjoint(ad, q);
+        }
     }
     // block sums of the parameter sensitivities (FP64)
 #pragma unroll
@@ -365,34 +374,24 @@ __global__ void __launch_bounds__(256) adjoint_transpose_z_kernel(AdjArgs<T> A, 
             }
         };
 
-        if (Q < 4) {
-            // lnrho and u: val, D1 x y z, lap, up x y z
-            const int b = Q == 0 ? G_LR : G_U + 8 * (Q - 1);
-            T r = ld(b + 0, 0, 0) - (d1x(b + 1) + d1y(b + 2));
-            r += sym(b + 4, 0, C.d2[0]) + sym(b + 4, 1, C.d2[1]);
-            if (UPWD)
-                r += sym(b + 5, 0, C.up[0]) + sym(b + 6, 1, C.up[1]);
+        // own stencils: first derivatives in x / y (absent for d_i A_i), the Laplacian, the
+        // upwind operators (lnrho and u); the z parts pushed into the chain
+        {
+            constexpr int kx = g_index(Q, ROLE_DX), ky = g_index(Q, ROLE_DY);
+            constexpr int kz = g_index(Q, ROLE_DZ), kl = g_index(Q, ROLE_LAP);
+            T r = sym(kl, 0, C.d2[0]) + sym(kl, 1, C.d2[1]);
+            if constexpr (kx >= 0)
+                r -= d1x(kx);
+            if constexpr (ky >= 0)
+                r -= d1y(ky);
+            if constexpr (UPWD && Q < 4)
+                r += sym(g_index(Q, ROLE_UPX), 0, C.up[0]) + sym(g_index(Q, ROLE_UPY), 1, C.up[1]);
             acc[3] += r;
-            push_anti(ld(b + 3, 0, 0));
-            push_sym(ld(b + 4, 0, 0), C.d2[2]);
-            if (UPWD)
-                push_sym(ld(b + 7, 0, 0), C.up[2]);
-        }
-        else if (Q < 7) {
-            // A: D1 x y z, lap
-            const int b = G_A + 4 * (Q - 4);
-            acc[3] += sym(b + 3, 0, C.d2[0]) + sym(b + 3, 1, C.d2[1]) - (d1x(b + 0) + d1y(b + 1));
-            push_anti(ld(b + 2, 0, 0));
-            push_sym(ld(b + 3, 0, 0), C.d2[2]);
-        }
-        else {
-            // ss: val, D1 x y z, lap
-            const int b = G_S;
-            T r = ld(b, 0, 0) - (d1x(b + 1) + d1y(b + 2));
-            r += sym(b + 4, 0, C.d2[0]) + sym(b + 4, 1, C.d2[1]);
-            acc[3] += r;
-            push_anti(ld(b + 3, 0, 0));
-            push_sym(ld(b + 4, 0, 0), C.d2[2]);
+            if constexpr (kz >= 0)
+                push_anti(ld(kz, 0, 0));
+            push_sym(ld(kl, 0, 0), C.d2[2]);
+            if constexpr (UPWD && Q < 4)
+                push_sym(ld(g_index(Q, ROLE_UPZ), 0, 0), C.up[2]);
         }
         // grad(div v) parts of the vector components (u: Q = 1..3 from G_GDU, A: 4..6 from G_GDA)
         if (Q >= 1 && Q <= 6) {
@@ -417,11 +416,7 @@ __global__ void __launch_bounds__(256) adjoint_transpose_z_kernel(AdjArgs<T> A, 
         if (it >= 2 * R) {
             const long long ob = sz * (long long)p * (long long)sizeof(T);  // array plane p - 3 + R
             T* lfp = (T*)((char*)A.lf + Q * nb + ob + colb);
-            T* lwp = (T*)((char*)A.lw + Q * nb + ob + colb);
-            const T lfo = *lfp;
-            const T mu = *lwp + a.beta * lfo;
-            *lfp = lfo + a.dt * acc[0];
-            *lwp = a.alpha * mu;
+            *lfp = *lfp + a.dt * acc[0];
         }
 #pragma unroll
         for (int k = 0; k < 6; ++k)
@@ -437,61 +432,71 @@ __global__ void __launch_bounds__(256) adjoint_transpose_z_kernel(AdjArgs<T> A, 
 // stages its planes), fields read only at the column come straight from memory, and the z parts
 // use the same push chain.
 
-// Fields of output Q that are read with in-plane neighbours ("staged"), in slot order.
+// Fields of output Q that are read with in-plane neighbours ("staged"), in slot order: its own
+// x / y first derivatives (those that exist), Laplacian and, for lnrho and u with upwinding, the
+// x / y upwind operators; then the grad(div) fields G0, G1[, G2] of the vector (G2 only for the
+// x and y components).
+__host__ __device__ constexpr int own_staged_role(int Q, bool upwd, int slot)
+{
+    int n = 0;
+    for (int role : {ROLE_DX, ROLE_DY, ROLE_LAP, ROLE_UPX, ROLE_UPY}) {
+        if ((role == ROLE_UPX || role == ROLE_UPY) && !(upwd && Q < 4))
+            continue;
+        if (g_index(Q, role) < 0)
+            continue;
+        if (n == slot)
+            return role;
+        ++n;
+    }
+    return -1;
+}
+__host__ __device__ constexpr int own_staged_count(int Q, bool upwd)
+{
+    int n = 0;
+    while (own_staged_role(Q, upwd, n) >= 0)
+        ++n;
+    return n;
+}
+// Slot of an own role among the staged fields of Q, or -1.
+__host__ __device__ constexpr int staged_slot(int Q, bool upwd, int role)
+{
+    for (int sl = 0; sl < own_staged_count(Q, upwd); ++sl)
+        if (own_staged_role(Q, upwd, sl) == role)
+            return sl;
+    return -1;
+}
 __host__ __device__ constexpr int staged_count(int Q, bool upwd)
 {
-    if (Q == 0)
-        return upwd ? 5 : 3;
-    if (Q <= 3)
-        return (upwd ? 5 : 3) + (Q == 3 ? 2 : 3);
-    if (Q <= 6)
-        return 3 + (Q == 6 ? 2 : 3);
-    return 3;
+    const int gd = (Q >= 1 && Q <= 6) ? ((Q - 1) % 3 == 2 ? 2 : 3) : 0;
+    return own_staged_count(Q, upwd) + gd;
 }
 __host__ __device__ constexpr int staged_field(int Q, bool upwd, int slot)
 {
-    // lnrho / u: D1x, D1y, lap[, up x, up y]; A: D1x, D1y, lap; ss: D1x, D1y, lap;
-    // then the grad(div) fields G0, G1[, G2] of the vector (G2 only for the x and y components)
-    if (Q <= 3) {
-        const int b = Q == 0 ? G_LR : G_U + 8 * (Q - 1);
-        const int own[5] = {b + 1, b + 2, b + 4, b + 5, b + 6};
-        const int no = upwd ? 5 : 3;
-        if (slot < no)
-            return own[slot];
-        return G_GDU + (slot - no);
-    }
-    if (Q <= 6) {
-        const int b = G_A + 4 * (Q - 4);
-        const int own[3] = {b + 0, b + 1, b + 3};
-        if (slot < 3)
-            return own[slot];
-        return G_GDA + (slot - 3);
-    }
-    const int own[3] = {G_S + 1, G_S + 2, G_S + 4};
-    return own[slot];
+    const int no = own_staged_count(Q, upwd);
+    if (slot < no)
+        return g_index(Q, own_staged_role(Q, upwd, slot));
+    return (Q <= 3 ? G_GDU : G_GDA) + (slot - no);
 }
 
-// Fields of output Q read only at the column itself, in slot order.
+// Fields of output Q read only at the column itself, in slot order: its own z first derivative
+// (where it exists) and z upwind operator (lnrho and u with upwinding); then G2 of grad(div) for
+// the z components.
+__host__ __device__ constexpr int own_column_count(int Q, bool upwd)
+{
+    return (g_index(Q, ROLE_DZ) >= 0 ? 1 : 0) + ((upwd && Q < 4) ? 1 : 0);
+}
 __host__ __device__ constexpr int column_count(int Q, bool upwd)
 {
-    if (Q <= 3)
-        return (upwd ? 3 : 2) + (Q == 3 ? 1 : 0);
-    if (Q <= 6)
-        return 1 + (Q == 6 ? 1 : 0);
-    return 2;
+    return own_column_count(Q, upwd) + ((Q == 3 || Q == 6) ? 1 : 0);
 }
 __host__ __device__ constexpr int column_field(int Q, bool upwd, int slot)
 {
-    // lnrho / u: val, D1z[, up z]; A: D1z; ss: val, D1z; then G2 of grad(div) for the z components
-    if (Q <= 3) {
-        const int b = Q == 0 ? G_LR : G_U + 8 * (Q - 1);
-        const int own[3] = {b + 0, b + 3, b + 7};
-        const int no = upwd ? 3 : 2;
-        return slot < no ? own[slot] : G_GDU + 2;
-    }
-    if (Q <= 6)
-        return slot == 0 ? G_A + 4 * (Q - 4) + 2 : G_GDA + 2;
-    return slot == 0 ? G_S : G_S + 3;
+    const bool dz = g_index(Q, ROLE_DZ) >= 0;
+    if (dz && slot == 0)
+        return g_index(Q, ROLE_DZ);
+    if (slot < own_column_count(Q, upwd))
+        return g_index(Q, ROLE_UPZ);
+    return (Q <= 3 ? G_GDU : G_GDA) + 2;
 }
 
 // TY = 16 (FP32) or 8 where ny allows (1024- or 512-thread workgroups: the staged in-plane halo
@@ -580,9 +585,9 @@ __global__ void __launch_bounds__(64 * TY) adjoint_transpose_lds_kernel(AdjArgs<
         acc[k] = T(0);
     // column values of the plane being processed, and of the next one (loaded a plane ahead)
     T col[NC], coln[NC];
-    // adjoint state of the plane completed in this step (lfc, lwc) and in the next one (loaded a
-    // plane ahead)
-    T lfc = T(0), lwc = T(0), lfn = T(0), lwn = T(0);
+    // lambda_f of the plane completed in this step (lfc) and in the next one (loaded a plane
+    // ahead); the local stage has already added its pointwise part
+    T lfc = T(0), lfn = T(0);
     auto load_cols = [&](int p, long long nb, unsigned colb, T* dst) {
         const char* pb = (const char*)A.g + plane_b(p);
 #pragma unroll
@@ -599,8 +604,7 @@ __global__ void __launch_bounds__(64 * TY) adjoint_transpose_lds_kernel(AdjArgs<
         dma_plane(z0 - R, 0, nb);
         load_cols(z0 - R, nb, colb0, col);
     }
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
+    plane_barrier();
     const int np = zchunk + 2 * R;
     for (int it = 0; it < np; ++it) {
         const int p = z0 - R + it;
@@ -613,7 +617,6 @@ __global__ void __launch_bounds__(64 * TY) adjoint_transpose_lds_kernel(AdjArgs<
             if (it + 1 >= 2 * R) {
                 const long long ob = out_off(it + 1, nb, colb);
                 lfn = *(const T*)((const char*)A.lf + ob);
-                lwn = *(const T*)((const char*)A.lw + ob);
             }
         }
         auto lds_at = [&](int sl, int dx, int dy) -> T {
@@ -669,34 +672,30 @@ __global__ void __launch_bounds__(64 * TY) adjoint_transpose_lds_kernel(AdjArgs<
             }
         };
 
-        // own stencils: staged slots 0: D1x, 1: D1y, 2: lap[, 3: up x, 4: up y]; column slots
-        // in column_field order
-        if (Q < 4) {
-            T r = col[0] - (d1x(0) + d1y(1));
-            r += sym(2, 0, C.d2[0]) + sym(2, 1, C.d2[1]);
-            if (UPWD)
-                r += sym(3, 0, C.up[0]) + sym(4, 1, C.up[1]);
+        // own stencils (staged and column slots as listed above)
+        {
+            constexpr int sx = staged_slot(Q, UPWD, ROLE_DX), sy_ = staged_slot(Q, UPWD, ROLE_DY);
+            constexpr int sl = staged_slot(Q, UPWD, ROLE_LAP);
+            constexpr int sux = staged_slot(Q, UPWD, ROLE_UPX), suy = staged_slot(Q, UPWD, ROLE_UPY);
+            constexpr bool has_dz = g_index(Q, ROLE_DZ) >= 0;
+            constexpr int cuz = has_dz ? 1 : 0;  // column slot of the z upwind operator
+            T r = sym(sl, 0, C.d2[0]) + sym(sl, 1, C.d2[1]);
+            if constexpr (sx >= 0)
+                r -= d1x(sx);
+            if constexpr (sy_ >= 0)
+                r -= d1y(sy_);
+            if constexpr (sux >= 0)
+                r += sym(sux, 0, C.up[0]) + sym(suy, 1, C.up[1]);
             acc[3] += r;
-            push_anti(col[1]);
-            push_sym(lds_at(2, 0, 0), C.d2[2]);
-            if (UPWD)
-                push_sym(col[2], C.up[2]);
-        }
-        else if (Q < 7) {
-            acc[3] += sym(2, 0, C.d2[0]) + sym(2, 1, C.d2[1]) - (d1x(0) + d1y(1));
-            push_anti(col[0]);
-            push_sym(lds_at(2, 0, 0), C.d2[2]);
-        }
-        else {
-            T r = col[0] - (d1x(0) + d1y(1));
-            r += sym(2, 0, C.d2[0]) + sym(2, 1, C.d2[1]);
-            acc[3] += r;
-            push_anti(col[1]);
-            push_sym(lds_at(2, 0, 0), C.d2[2]);
+            if constexpr (has_dz)
+                push_anti(col[0]);
+            push_sym(lds_at(sl, 0, 0), C.d2[2]);
+            if constexpr (sux >= 0)
+                push_sym(col[cuz], C.up[2]);
         }
         // grad(div v) parts (staged slots from the own-stencil count: G0, G1[, G2])
         if (Q >= 1 && Q <= 6) {
-            constexpr int no = Q <= 3 ? (UPWD ? 5 : 3) : 3;
+            constexpr int no = own_staged_count(Q, UPWD);
             const int comp = (Q - 1) % 3;
             if (comp == 0) {
                 acc[3] += sym(no + 0, 0, C.d2[0]) + mixed_xy(no + 1);
@@ -715,10 +714,7 @@ __global__ void __launch_bounds__(64 * TY) adjoint_transpose_lds_kernel(AdjArgs<
 
         if (it >= 2 * R) {
             const long long ob = out_off(it, nb, colb);
-            const T lfo = lfc;
-            const T mu = lwc + a.beta * lfo;
-            *(T*)((char*)A.lf + ob) = lfo + a.dt * acc[0];
-            *(T*)((char*)A.lw + ob) = a.alpha * mu;
+            *(T*)((char*)A.lf + ob) = lfc + a.dt * acc[0];
         }
 #pragma unroll
         for (int k = 0; k < 6; ++k)
@@ -728,26 +724,9 @@ __global__ void __launch_bounds__(64 * TY) adjoint_transpose_lds_kernel(AdjArgs<
         for (int c = 0; c < NC; ++c)
             col[c] = coln[c];
         lfc = lfn;
-        lwc = lwn;
         // plane p + 1 has landed and every wave is done with this one
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
+        plane_barrier();
     }
-}
-
-// Without the entropy equation ss is not evolved (f'_7 = f_7): its adjoint passes through and
-// lambda_w of ss is zero.
-template <typename T>
-__global__ void __launch_bounds__(256) adjoint_ss_passive_kernel(AdjArgs<T> A)
-{
-    const SolveArgs<T>& a = A.s;
-    const int i = threadIdx.x + blockIdx.x * blockDim.x;
-    const int j = threadIdx.y + blockIdx.y * blockDim.y;
-    const int k = blockIdx.z;
-    if (i >= a.nx || j >= a.ny || k >= a.nz)
-        return;
-    const long long c = (long long)(i + R) + (long long)a.mx * (j + R) + a.sz * (k + R);
-    A.lw[7 * A.count + c] = T(0);
 }
 
 // ---------------------------------------------------------------- launchers
@@ -845,10 +824,10 @@ void launch_adjoint(const AdjArgs<T>& A, bool upwd, bool entropy, double* param_
     AC_ADJ_Q(4);
     AC_ADJ_Q(5);
     AC_ADJ_Q(6);
+    // without the entropy equation ss is not evolved (f'_7 = f_7): its lambda_f passes through
+    // and the local stage has set its lambda_w to alpha * 0
     if (entropy)
         AC_ADJ_Q(7);
-    else
-        hipLaunchKernelGGL((adjoint_ss_passive_kernel<T>), bpg, tpb, 0, s, A);
 #undef AC_ADJ_Q
 #undef AC_ADJ
 }

@@ -57,12 +57,14 @@ bool sweep_shape_fits(TiledShape s, int nx, int ny, int nz);
 template <typename T>
 bool sweep_applies(const SolveArgs<T>& a, int kahan_bits, TiledShape shape);
 // The adjoint's local stage on the tiled machinery: derivative adjoints g (NG fields of m^3,
-// same field stride as the state), lambda_f / lambda_w (NF fields), (nu, eta) block partials.
+// same field stride as the state), lambda_f / lambda_w (NF fields, updated in place at the
+// interior cells: lambda_w <- alpha mu, lambda_f <- lambda_f + dt * the value adjoints), (nu, eta)
+// block partials.
 template <typename T>
 struct AdjLocal {
     T* g;
-    const T* lf;
-    const T* lw;
+    T* lf;
+    T* lw;
     double* partials;
 };
 // Returns false (nothing launched) when the grid does not fit 64 x 4 tiles.
@@ -72,6 +74,21 @@ bool launch_adjoint_local_tiled(const SolveArgs<T>& a, const AdjLocal<T>& x, boo
 
 // LDS pointer type of the global->LDS DMA builtin
 typedef __attribute__((address_space(3))) void* lds_ptr_t;
+
+// End of a plane step of the LDS-staged z-marching kernels: wait for this wave's memory
+// operations (its DMA into the other plane buffer among them), then the workgroup barrier.
+// The global->LDS DMA builtin writes LDS at M0 + inst_offset + 4 * lane, a range the optimiser
+// does not see (its pointer operand is the wave's base). Without the memory clobbers around the
+// barrier it may keep an LDS value read before the barrier in a register for a later read of
+// the same address, or issue a read after the barrier, when another wave's DMA of the next
+// plane can already have overwritten it; the clobbers pin every LDS access to its own side.
+__device__ __forceinline__ void plane_barrier()
+{
+    __builtin_amdgcn_s_waitcnt(0);
+    asm volatile("" ::: "memory");
+    __syncthreads();
+    asm volatile("" ::: "memory");
+}
 
 // Returns v unchanged, but opaque to the optimiser. Used on per-lane 32-bit offsets inside the
 // plane loops: otherwise loop-invariant code motion folds them into 64-bit per-lane addresses

@@ -11,15 +11,38 @@
 
 namespace acamd {
 
-// Layout of the derivative adjoints (the adjoint of every derivative quantity the right-hand
-// side reads), one m^3 field each, NG in all.
-constexpr int NG = 55;
-constexpr int G_LR = 0;   // lnrho: val, D1 x y z, lap, up x y z
-constexpr int G_U = 8;    // + 8 i, u_i: val, D1 x y z, lap, up x y z
-constexpr int G_GDU = 32; // grad(div u) x y z
-constexpr int G_A = 35;   // + 4 i, A_i: D1 x y z, lap
-constexpr int G_GDA = 47; // grad(div A) x y z
-constexpr int G_S = 50;   // ss: val, D1 x y z, lap
+// Layout of the derivative adjoints: the adjoint of every derivative quantity the right-hand
+// side reads, which the transposed stencils gather from the neighbours, one m^3 field each, NG
+// in all. The adjoints of a cell's own values (lnrho, u, ss) need no stencil: the local stage
+// adds them to lambda_f directly. The diagonal derivatives d_i A_i do not enter the right-hand
+// side (B = curl A, and grad(div A) comes from the second and mixed stencils), so they have no
+// field.
+//   lnrho (q = 0) and u_i (q = 1..3): d_x, d_y, d_z, lap, up_x, up_y, up_z  at 7 q
+//   grad(div u) x y z                                                        at G_GDU
+//   A_i (q = 4..6): the two off-diagonal first derivatives (x before y before z), lap
+//                                                                            at 31 + 3 (q - 4)
+//   grad(div A) x y z                                                        at G_GDA
+//   ss (q = 7): d_x, d_y, d_z, lap                                           at 43
+enum : int { ROLE_DX = 0, ROLE_DY = 1, ROLE_DZ = 2, ROLE_LAP = 3, ROLE_UPX = 4, ROLE_UPY = 5,
+             ROLE_UPZ = 6 };
+constexpr int NG = 47;
+constexpr int G_GDU = 28;
+constexpr int G_GDA = 40;
+// Field index of the adjoint of derivative `role` of field q, or -1 when there is none.
+__host__ __device__ constexpr int g_index(int q, int role)
+{
+    if (q <= 3)
+        return 7 * q + role;
+    if (q <= 6) {
+        const int i = q - 4;
+        if (role == ROLE_LAP)
+            return 31 + 3 * i + 2;
+        if (role > ROLE_DZ || role == i)
+            return -1;
+        return 31 + 3 * i + (role < i ? role : role - 1);
+    }
+    return role <= ROLE_LAP ? 43 + role : -1;
+}
 
 // Upwind derivatives per direction: ul[j] = up_j(lnrho), uu[i][j] = up_j(u_i).
 template <typename T>
@@ -422,35 +445,44 @@ __device__ __forceinline__ void mhd_rhs_vjp(const CellDerivs<T>& d, const Upwind
     }
 }
 
-// Hands the derivative adjoints of one cell to put(field index, value) in the layout above.
-template <typename T, typename Put>
+// Hands the derivative adjoints of one cell to put(field index, value) in the layout above
+// (the upwind ones only with upwinding: without it no transposed stencil reads them).
+template <bool UPWD, typename T, typename Put>
 __device__ __forceinline__ void put_derivative_adjoints(const CellDerivs<T>& ad,
                                                         const UpwindDirs<T>& aup, Put&& put)
 {
-    put(G_LR + 0, ad.lr);
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
-        put(G_LR + 1 + ax, ad.glr[ax]);
-        put(G_LR + 5 + ax, aup.ul[ax]);
+        put(g_index(0, ROLE_DX + ax), ad.glr[ax]);
+        if (UPWD)
+            put(g_index(0, ROLE_UPX + ax), aup.ul[ax]);
+        put(g_index(7, ROLE_DX + ax), ad.gs[ax]);
     }
-    put(G_LR + 4, ad.laplr);
+    put(g_index(0, ROLE_LAP), ad.laplr);
+    put(g_index(7, ROLE_LAP), ad.laps);
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-        put(G_U + 8 * i + 0, ad.u[i]);
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax) {
-            put(G_U + 8 * i + 1 + ax, ad.gu[i][ax]);
-            put(G_U + 8 * i + 5 + ax, aup.uu[i][ax]);
-            put(G_A + 4 * i + ax, ad.ga[i][ax]);
+            put(g_index(1 + i, ROLE_DX + ax), ad.gu[i][ax]);
+            if (UPWD)
+                put(g_index(1 + i, ROLE_UPX + ax), aup.uu[i][ax]);
+            if (ax != i)
+                put(g_index(4 + i, ROLE_DX + ax), ad.ga[i][ax]);
         }
-        put(G_U + 8 * i + 4, ad.lapu[i]);
-        put(G_A + 4 * i + 3, ad.lapa[i]);
+        put(g_index(1 + i, ROLE_LAP), ad.lapu[i]);
+        put(g_index(4 + i, ROLE_LAP), ad.lapa[i]);
         put(G_GDU + i, ad.gdu[i]);
         put(G_GDA + i, ad.gda[i]);
-        put(G_S + 1 + i, ad.gs[i]);
     }
-    put(G_S + 0, ad.s);
-    put(G_S + 4, ad.laps);
+}
+
+// The adjoints of the cell's own values, per field (zero for A, whose values the right-hand
+// side does not read).
+template <typename T>
+__device__ __forceinline__ T value_adjoint(const CellDerivs<T>& ad, int q)
+{
+    return q == 0 ? ad.lr : (q <= 3 ? ad.u[q - 1] : (q == 7 ? ad.s : T(0)));
 }
 
 }  // namespace acamd

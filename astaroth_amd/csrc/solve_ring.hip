@@ -129,8 +129,7 @@ __global__ void __launch_bounds__(TX* TY, (sizeof(T) == 4 ? 2 : 1))
 #pragma unroll
     for (int k = 0; k < NSLOT; ++k)
         dma_plane(z0 - R + k, k, a.fstride * (long long)sizeof(T));
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
+    plane_barrier();
 
     const int niter = zchunk + R;
     for (int it = 0; it < niter; ++it) {
@@ -294,8 +293,7 @@ __global__ void __launch_bounds__(TX* TY, (sizeof(T) == 4 ? 2 : 1))
         __builtin_amdgcn_sched_barrier(0);
 
         // ================= plane z+3 has landed everywhere; slot s0 is free for plane z+4
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
+        plane_barrier();
         if (it + 1 < niter)
             dma_plane(z + 4, s0, fsb);
         __builtin_amdgcn_sched_barrier(0);

@@ -204,8 +204,7 @@ __device__ __forceinline__ void sweep_body(const SolveArgs<T>& a, int zchunk, in
     const int niter = zchunk + 2 * R;
 
     dma_plane(lds0, z0 - R, a.fstride * 4);
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
+    plane_barrier();
 
     for (int it0 = 0;; it0 += UNR) {
         auto plane_step = [&](auto UC) __attribute__((always_inline)) {
@@ -532,8 +531,7 @@ __device__ __forceinline__ void sweep_body(const SolveArgs<T>& a, int zchunk, in
             }
 
             // plane p + 1 has landed in the other buffer and every wave is done with this one
-            __builtin_amdgcn_s_waitcnt(0);
-            __syncthreads();
+            plane_barrier();
         };
         plane_step(std::integral_constant<int, 0>{});
         if (it0 + 1 >= niter)

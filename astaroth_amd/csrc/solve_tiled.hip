@@ -205,8 +205,7 @@ __device__ __forceinline__ void delay_line_body(const SolveArgs<T>& a, const Adj
     dma_plane(z0 - R, 0, a.fstride * (long long)sizeof(T));
     if (LATE)
         dma_plane(z0 - R + 1, 1, a.fstride * (long long)sizeof(T));
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
+    plane_barrier();
     const int niter = zchunk + 2 * R;
     for (int it = 0; it < niter; ++it) {
         const int p = z0 - R + it;
@@ -402,9 +401,19 @@ __device__ __forceinline__ void delay_line_body(const SolveArgs<T>& a, const Adj
                 pnu += (double)dt * (double)anu;
                 peta += (double)dt * (double)aeta;
                 const long long poffb = sz * (long long)(z + R) * (long long)sizeof(T);
-                put_derivative_adjoints(ad, aup, [&](int k, T v) {
+                put_derivative_adjoints<UPWD>(ad, aup, [&](int k, T v) {
                     st_b((T*)((char*)x.g + (k * fsb + poffb)), iob, v);
                 });
+                // the pointwise parts of the substep's adjoint: lambda_w <- alpha mu, and
+                // lambda_f gains dt times the adjoint of the cell's own value (the transposed
+                // stencils add the rest)
+#pragma unroll
+                for (int f = 0; f < NF; ++f) {
+                    st_b((T*)((char*)x.lw + (f * fsb + poffb)), iob, a.alpha * mu[f]);
+                    if (f < 4 || (f == 7 && ENTROPY))
+                        st_b((T*)((char*)x.lf + (f * fsb + poffb)), iob,
+                             lfpre[f] + dt * value_adjoint(ad, f));
+                }
             }
             else {
             T rhs[NF];
@@ -564,8 +573,7 @@ __device__ __forceinline__ void delay_line_body(const SolveArgs<T>& a, const Adj
             // every wave is done with buffer cur (plane p) and plane p + 1 has landed in the
             // other one; the DMA of plane p + 2 into cur then overlaps the output work, and the
             // output's stores have the next step's stencil work to complete before its barrier
-            __builtin_amdgcn_s_waitcnt(0);
-            __syncthreads();
+            plane_barrier();
             if (it + 2 < niter)
                 dma_plane(p + 2, cur, fsb);
             if (it >= 2 * R)
@@ -594,8 +602,7 @@ __device__ __forceinline__ void delay_line_body(const SolveArgs<T>& a, const Adj
 
         if constexpr (!LATE) {
             // plane p + 1 has landed in the other buffer and every wave is done with this one
-            __builtin_amdgcn_s_waitcnt(0);
-            __syncthreads();
+            plane_barrier();
         }
     }
     if constexpr (ADJ) {

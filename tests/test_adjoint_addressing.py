@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 R = 3
-NG = 55  # derivative-adjoint fields in the workspace (mhd_adjoint.h)
+NG = 47  # derivative-adjoint fields in the workspace (mhd_adjoint.h)
 OFFSETS = sorted({(d, 0) for d in range(-3, 4)} | {(0, d) for d in range(-3, 4)} |
                  {(d, d) for d in range(-3, 4)} | {(d, -d) for d in range(-3, 4)})
 

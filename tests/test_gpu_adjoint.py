@@ -90,7 +90,10 @@ def substep_dot_product(lib, info, f, step, entropy, seed):
                                                      df_out.data_ptr(), dw_t.data_ptr(), dparams),
                  "tangent")
         dev.sync()
-        ws = torch.empty(int(lib.acAdjointWorkspaceBytes(dev.handle)), dtype=torch.uint8, device=cuda)
+        # the workspace starts as NaN bit patterns: a read of any part the adjoint did not write
+        # first turns the result into NaN
+        ws = torch.full((int(lib.acAdjointWorkspaceBytes(dev.handle)),), 0xFF, dtype=torch.uint8,
+                        device=cuda)
         pg = torch.zeros(2, dtype=torch.float64, device=cuda)
         lf_a, lw_a = lf.clone(), lw.clone()
         torch.cuda.synchronize()
@@ -122,6 +125,53 @@ def test_substep_dot_product(lib, step, upwd, entropy, n):
     assert rel < tol
     # the parameter sensitivities alone: dot-product with dx = 0
     assert float(pg.abs().sum()) > 0
+
+
+@pytest.mark.parametrize("upwd,entropy", [(True, True), (False, False), (True, False)])
+@pytest.mark.parametrize("n", [16, (64, 8, 16), (64, 12, 16), (64, 16, 16)])
+def test_adjoint_substep_deterministic(lib, upwd, entropy, n):
+    """The adjoint substep on identical inputs gives identical outputs, bit for bit, every time.
+    Regression test: the LDS-staged transposes once read a value of one plane buffer after the
+    step barrier, when another wave's DMA of the next plane could already have refilled it (the
+    optimiser had moved the read; the kernels now fence LDS accesses at the barrier), and the
+    result then varied between runs."""
+    say(f"case adjoint determinism {lib.precision} upwd={upwd} entropy={entropy} n={n}")
+    info = physics_info(lib, n, AC_lupwd=upwd, AC_lentropy=entropy)
+    cuda = torch.device("cuda", 0)
+    tdt = torch.float32 if lib.precision == "fp32" else torch.float64
+    for step in range(3):
+        f = initial(lib, info, 3 + step)
+        dev = ac.Device(lib, info)
+        try:
+            ft = torch.from_numpy(f).to(cuda)
+            torch.cuda.synchronize()
+            ac.check(lib.acDeviceLoadMeshFromDevicePtr(dev.handle, 0, ft.data_ptr()), "load")
+            dev.periodic()
+            dev.sync()
+            g = torch.Generator(device="cpu").manual_seed(11 + step)
+            lf = torch.randn(f.shape, generator=g, dtype=torch.float64).to(tdt).to(cuda)
+            lw = torch.randn(f.shape, generator=g, dtype=torch.float64).to(tdt).to(cuda)
+            ws = torch.full((int(lib.acAdjointWorkspaceBytes(dev.handle)),), 0xFF,
+                            dtype=torch.uint8, device=cuda)
+            outs = []
+            for _ in range(6):
+                lf_a, lw_a = lf.clone(), lw.clone()
+                pg = torch.zeros(2, dtype=torch.float64, device=cuda)
+                torch.cuda.synchronize()
+                ac.check(lib.acDeviceIntegrateSubstepAdjoint(dev.handle, 0, step, lib.np_real(0.5),
+                                                             lf_a.data_ptr(), lw_a.data_ptr(),
+                                                             ws.data_ptr(), pg.data_ptr()),
+                         "adjoint")
+                dev.sync()
+                outs.append((interior(lf_a).cpu(), interior(lw_a).cpu(), pg.cpu()))
+            for o in outs[1:]:
+                for q in range(ac.NUM_VTXBUF_HANDLES):
+                    assert torch.equal(o[0][q], outs[0][0][q]), ("lambda_f", step, q)
+                    assert torch.equal(o[1][q], outs[0][1][q]), ("lambda_w", step, q)
+                assert torch.equal(o[2], outs[0][2]), ("params", step)
+            assert torch.isfinite(outs[0][0]).all() and torch.isfinite(outs[0][1]).all()
+        finally:
+            dev.close()
 
 
 def loss_and_target(lib, n, upwd=True):

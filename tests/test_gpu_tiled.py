@@ -195,3 +195,38 @@ def test_fused_periodic_steps_kahan(n, shape, kb, require_gpu):
     lib = ac.load("fp32")
     for mode in ("courant", "reference"):
         test_fused_periodic_steps(lib, n, shape, mode, kb=kb)
+
+
+@pytest.mark.parametrize("shape,kind", [((64, 4, 16), 0), ((64, 2, 16), 0), ((32, 8, 16), 0),
+                                        ((16, 16, 16), 0), ((64, 4, 16), 1)])
+@pytest.mark.parametrize("kb", [0, 16])
+def test_solve_deterministic(lib, shape, kind, kb):
+    """The solve on identical inputs writes identical outputs, bit for bit, run after run, on the
+    multi-step path (periodic images read by the kernel) for every kernel kind. Regression test
+    for LDS reads that the optimiser could move across the plane barrier (see
+    test_adjoint_substep_deterministic)."""
+    if kb and lib.precision == "fp64":
+        pytest.skip("compensation is an FP32 mode")
+    if kind == 1 and lib.precision == "fp64":
+        pytest.skip("the sweep kernel is FP32")
+    skip_ring_fp64(lib, shape)
+    n = (64, 32, 32)
+    say(f"case solve determinism {lib.precision} shape={shape} kind={kind} kb={kb}")
+    info = physics_info(lib, n, shape, AC_solve_kernel=kind, AC_kahan_bits=kb)
+    data = initial(lib, info, seed=17)
+    outs = []
+    for _ in range(4):
+        mesh = ac.HostMesh(lib, info, data)
+        with ac.Device(lib, info) as dev:
+            dev.load_mesh(mesh)
+            dev.periodic()
+            ac.check(lib.acDeviceUpdateTimestepCFL(dev.handle, 0, lib.np_real(0.3)), "cfl")
+            ac.check(lib.acDeviceIntegrateStepsCFL(dev.handle, 0, 2, lib.np_real(0.3), None),
+                     "steps")
+            dev.periodic()
+            out = ac.HostMesh(lib, info)
+            dev.store_mesh(out)
+            outs.append(out.data.copy())
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+    assert np.isfinite(outs[0]).all()
