@@ -148,6 +148,7 @@ static void make_phys(const AcMeshInfo& info, PhysParams<AcReal>& P)
     P.gamma_inv_cp = (AcReal)(gamma / cp);
     P.eta_mu0 = (AcReal)((double)info.real_params[AC_eta] * mu0);
     P.nu_third = (AcReal)((double)info.real_params[AC_nu_visc] / 3.0);
+    P.pT_scale = (AcReal)(cs * cs * exp(-(double)info.real_params[AC_lnT0]));
 }
 
 // Parameters of the device timestep formula (cfl_dt) for the device's timestep mode. In the
