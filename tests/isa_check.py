@@ -13,6 +13,18 @@ from the kernel entry:
 A read of never-written storage is what a value that depends on whatever the GPU ran before
 looks like in the machine code. Reads through a dynamic lane index (v_readlane with a scalar lane
 operand, as in cross-lane reductions) are not spill traffic and are not tracked.
+
+`wait_state_violations(lines)` checks the wait states that software (the compiler) must place
+between dependent instructions on this target family, on every control-flow path:
+
+  * a VALU instruction writes a scalar register (v_readlane / v_readfirstlane to an SGPR, the
+    SGPR result of a compare or of a carry-out) and a vector memory instruction reads it as an
+    address, resource or offset: 5 wait states;
+  * the same write followed by v_readlane / v_writelane using it as the lane select: 4;
+  * a scalar write of M0 followed by an LDS DMA (global_load_lds / buffer_load ... lds), which
+    takes its LDS address from M0: 1.
+
+Each instruction in between counts one wait state, s_nop N counts N + 1.
 """
 import collections
 import re
@@ -152,4 +164,113 @@ def undefined_reads(lines):
             for x in sorted(u - s):
                 found.append((label, x, t))
             s |= d
+    return found
+
+
+def _sregs(tok):
+    m = re.match(r"^s\[(\d+):(\d+)\]$", tok)
+    if m:
+        return {f"s{i}" for i in range(int(m.group(1)), int(m.group(2)) + 1)}
+    if re.match(r"^s\d+$", tok):
+        return {tok}
+    if tok == "vcc":
+        return {"vcc_lo", "vcc_hi"}
+    if tok in ("vcc_lo", "vcc_hi", "m0"):
+        return {tok}
+    return set()
+
+
+def _toks(t):
+    return [x for x in t.replace(",", " ").split() if x]
+
+
+def _valu_sgpr_defs(t):
+    ts = _toks(t)
+    op = ts[0]
+    if not op.startswith("v_"):
+        return set()
+    if op.startswith(("v_readlane", "v_readfirstlane")):
+        return _sregs(ts[1])
+    if op.startswith(("v_cmp", "v_cmpx")):
+        return _sregs(ts[1]) if op.endswith("_e64") else {"vcc_lo", "vcc_hi"}
+    if op.startswith(("v_add_co", "v_sub_co", "v_subrev_co", "v_addc_co", "v_subb_co",
+                      "v_subbrev_co", "v_mad_u64_u32", "v_mad_i64_i32", "v_div_scale")):
+        d = _sregs(ts[2]) if len(ts) > 2 else set()
+        if op.endswith("_e32"):
+            d |= {"vcc_lo", "vcc_hi"}
+        return d
+    return set()
+
+
+def _salu_m0_def(t):
+    ts = _toks(t)
+    return {"m0"} if ts[0].startswith("s_") and len(ts) > 1 and ts[1] == "m0" else set()
+
+
+def _wait_states(t):
+    ts = _toks(t)
+    return int(ts[1], 0) + 1 if ts[0] == "s_nop" else 1
+
+
+def _cfg_preds(blocks):
+    index = {label: i for i, (label, _) in enumerate(blocks)}
+    pred = collections.defaultdict(list)
+    for i, (_, ins) in enumerate(blocks):
+        last = ins[-1] if ins else ""
+        op = last.split()[0] if last else ""
+        if op.startswith("s_cbranch"):
+            pred[index[last.split()[-1]]].append(i)
+            if i + 1 < len(blocks):
+                pred[i + 1].append(i)
+        elif op == "s_branch":
+            pred[index[last.split()[-1]]].append(i)
+        elif op.startswith(("s_endpgm", "s_setpc")):
+            pass
+        elif i + 1 < len(blocks):
+            pred[i + 1].append(i)
+    return pred
+
+
+def wait_state_violations(lines):
+    """List of (rule, wait states found, producer, consumer, block label) for one kernel body."""
+    blocks = _blocks(lines)
+    pred = _cfg_preds(blocks)
+
+    def back(bi, pos, regs, need, defs_of, acc, seen):
+        # most recent producer of `regs` on every path back from instruction pos of block bi
+        ins = blocks[bi][1]
+        for j in range(pos - 1, -1, -1):
+            if defs_of(ins[j]) & regs:
+                return [(acc, ins[j])] if acc < need else []
+            acc += _wait_states(ins[j])
+            if acc >= need:
+                return []
+        out = []
+        for p in pred[bi]:
+            if (p, acc) not in seen:
+                seen.add((p, acc))
+                out += back(p, len(blocks[p][1]), regs, need, defs_of, acc, seen)
+        return out
+
+    found = []
+    for bi, (label, ins) in enumerate(blocks):
+        for k, t in enumerate(ins):
+            ts = _toks(t)
+            op = ts[0]
+            checks = []
+            if op.startswith(("global_", "buffer_", "scratch_", "flat_")):
+                u = set()
+                for x in ts[1:]:
+                    u |= _sregs(x)
+                u.discard("m0")
+                if u:
+                    checks.append(("VALU-SGPR->VMEM", u, 5, _valu_sgpr_defs))
+            if op.startswith(("v_readlane", "v_writelane")) and len(ts) > 3 and _sregs(ts[3]):
+                checks.append(("VALU-SGPR->lane-select", _sregs(ts[3]), 4, _valu_sgpr_defs))
+            if op.startswith("global_load_lds") or (op.startswith("buffer_load") and
+                                                     " lds" in t):
+                checks.append(("SALU-M0->LDS-DMA", {"m0"}, 1, _salu_m0_def))
+            for rule, regs, need, defs_of in checks:
+                for ws, prod in back(bi, k, regs, need, defs_of, 0, set()):
+                    found.append((rule, ws, prod, t, label))
     return found
