@@ -1235,13 +1235,15 @@ AcResult acDeviceGetSolveKernel(const Device d, int* kind)
 // ------------------------------------------------------------------ autotuner
 // Times every candidate launch (the direct kernel, the delay-line shapes and the sweep shapes) on
 // the current grid with the device's own compensation width, physics switches and periodic-image
-// reads, at substep 0 (reads f, writes f' and w), and keeps the fastest. No allocation: the trial
-// launches read the real state and write f' into the out slab, which holds nothing between
-// substeps. When the tuning runs ahead of a substep-0 launch (every caller's first substep), w is
-// dead as well -- that launch overwrites it without reading it -- and the trials write w into w,
-// so they move the same bytes as the real launches; otherwise (an explicit acDeviceAutotune) w
-// goes into the out slab too, aliasing f', which halves the trial's write traffic. The
-// compensation records (read, and written into the out slab) stay untouched. The choice can be
+// reads, and keeps the fastest. No allocation: the trial launches read the real state and write
+// f' into the out slab, which holds nothing between substeps. When the tuning runs ahead of a
+// substep-0 launch (every caller's first substep), w is dead as well -- that launch overwrites it
+// without reading it -- and a trial is one launch of each RK3 substep with w read from and
+// written into w, the same bytes and the same substep mix as the stepping (a kernel whose
+// substep 0 is fast but whose w-reading substeps are slow was picked when only substep 0 was
+// timed). Otherwise (an explicit acDeviceAutotune) a trial is substep 0 alone, which reads f
+// and writes f' and w, with w into the out slab too (aliasing f', which halves the trial's write
+// traffic). The compensation records (read, and written into the out slab) stay untouched. The choice can be
 // cached in the file named by ASTAROTH_AUTOTUNE_CACHE (lines "nx ny nz kahan upwd entropy
 // realsize tx ty zchunk kind"); ASTAROTH_AUTOTUNE_VERBOSE=1 prints each candidate's time to
 // stderr.
@@ -1307,20 +1309,26 @@ static AcResult autotune(Device d, hipStream_t s, bool w_dead)
     const bool entropy = d->local_config.bool_params[AC_lentropy] != 0;
     const int kb = d->kahan_bits;
     const bool verbose = getenv("ASTAROTH_AUTOTUNE_VERBOSE") != nullptr;
-    SolveArgs<AcReal> a;
-    fill_solve_args(d, a, 0, (AcReal)1e-9);
     // Trial outputs: f' into out, w into w when it is dead (else into out), the compensation
     // records (at most NF * KB / 8 <= NF * sizeof(float) bytes per cell) into out. Only a single
     // out slab (no field swapped alone) qualifies; otherwise only the direct kernel is timed.
+    // With w dead a trial is one launch of each RK3 substep (s = 1 and 2 read w, s = 0 and 1
+    // write it), the mix the stepping runs; with w live, substep 0 alone (it never reads w).
+    const int ntrial = w_dead ? 3 : 1;
+    SolveArgs<AcReal> ta[3];
     bool one_slab = true;
     for (int q = 0; q < NF; ++q)
         one_slab = one_slab && d->out[q] == d->out[0] + (size_t)q * d->count;
-    for (int q = 0; q < NF; ++q) {
-        if (!w_dead)
-            a.w[q] = a.out[q];
-        a.comp[q] = kb > 0 ? (void*)((char*)d->out[0] + (size_t)q * (kb / 8)) : nullptr;
+    for (int t = 0; t < ntrial; ++t) {
+        fill_solve_args(d, ta[t], t, (AcReal)1e-9);
+        for (int q = 0; q < NF; ++q) {
+            if (!w_dead)
+                ta[t].w[q] = ta[t].out[q];
+            ta[t].comp[q] = kb > 0 ? (void*)((char*)d->out[0] + (size_t)q * (kb / 8)) : nullptr;
+        }
+        ta[t].wrap = 1;
     }
-    a.wrap = 1;
+    const SolveArgs<AcReal>& a = ta[0];
     hipEvent_t e0, e1;
     AC_HIP_CHECK(hipEventCreate(&e0));
     if (hipEventCreate(&e1) != hipSuccess) {
@@ -1331,13 +1339,17 @@ static AcResult autotune(Device d, hipStream_t s, bool w_dead)
     // minimum is robust against another process's work or a clock ramp during one sample).
     auto time_candidate = [&](TiledShape sh) -> float {
         auto launch = [&]() -> bool {
-            if (sh.tx <= 0) {
-                SolveArgs<AcReal> ad = a;
-                ad.wrap = 0;  // the direct kernel reads the halos as stored
-                launch_solve_direct<AcReal>(ad, upwd, entropy, kb, s);
-                return true;
+            for (int t = 0; t < ntrial; ++t) {
+                if (sh.tx <= 0) {
+                    SolveArgs<AcReal> ad = ta[t];
+                    ad.wrap = 0;  // the direct kernel reads the halos as stored
+                    launch_solve_direct<AcReal>(ad, upwd, entropy, kb, s);
+                }
+                else if (!launch_shape(ta[t], upwd, entropy, kb, sh, s)) {
+                    return false;
+                }
             }
-            return launch_shape(a, upwd, entropy, kb, sh, s);
+            return true;
         };
         if (!launch())
             return 1e30f;
@@ -1356,7 +1368,8 @@ static AcResult autotune(Device d, hipStream_t s, bool w_dead)
         }
         if (verbose)
             fprintf(stderr, "astaroth_amd autotune: %d^3-ish %dx%dx%d kb=%d kind=%d: %.4f ms per launch\n",
-                    n.x, sh.tx, sh.ty, sh.zchunk, kb, sh.tx <= 0 ? -1 : sh.kind, best_ms / 2);
+                    n.x, sh.tx, sh.ty, sh.zchunk, kb, sh.tx <= 0 ? -1 : sh.kind,
+                    best_ms / (2 * ntrial));
         return best_ms;
     };
     std::vector<TiledShape> cands;

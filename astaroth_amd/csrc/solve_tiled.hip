@@ -113,6 +113,14 @@ __device__ __forceinline__ void delay_line_body(const SolveArgs<T>& a, const Adj
     // elements either way, and their assembly reads no undefined register, spill slot or lane.
     const int wave = sizeof(T) == 4 ? __builtin_amdgcn_readfirstlane(tid >> 6) : tid >> 6;
     const int lane = tid & 63;
+    // Two workgroups share each CU in FP32, one wave of each per SIMD, and the SIMD's arbiter
+    // favours the wave in slot 0 when both can issue: measured per workgroup, the slot-0 one
+    // finished about 7% earlier and the other ran its last planes alone, at half occupancy.
+    // The slot-1 wave takes the higher issue priority on every other plane step, so the two
+    // progress at about the same rate (256^3: 0.5846 -> 0.5652-0.5686 ms per launch). Slot =
+    // HW_ID bits 3:0, read once (wave-uniform).
+    constexpr bool PRIO = sizeof(T) == 4 && !ADJ;
+    const unsigned wslot = PRIO ? (__builtin_amdgcn_s_getreg((3 << 11) | 4) & 1u) : 0u;
     const int x0 = tile_x * TX, y0 = tile_y * TY, z0 = chunk * zchunk;
     const long long sy = a.mx, sz = a.sz;
 
@@ -210,6 +218,12 @@ __device__ __forceinline__ void delay_line_body(const SolveArgs<T>& a, const Adj
     for (int it = 0; it < niter; ++it) {
         const int p = z0 - R + it;
         const int cur = it & 1;
+        if constexpr (PRIO) {
+            if (wslot & (unsigned)it)
+                __builtin_amdgcn_s_setprio(1);
+            else
+                __builtin_amdgcn_s_setprio(0);
+        }
         const unsigned iob = opaque(ioff * (unsigned)sizeof(T));  // byte offset in a plane
         const long long fsb = opaque_s64(a.fstride * (long long)sizeof(T));  // field stride, bytes
         if (!LATE && it + 1 < niter)
