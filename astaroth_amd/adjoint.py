@@ -180,6 +180,9 @@ class MHDAdjoint:
 
     # ---- adjoint of one step whose input state is in the device
     def reverse_step(self, lam_f, lam_w):
+        # the states entering substeps 0 and 1 are saved; the one entering substep 2 is left in
+        # the device by the recomputation, where the first adjoint substep reads it (an adjoint
+        # substep reads the device state and writes only lambda and its workspace)
         s0 = self.empty()
         self.store(s0)
         states = [s0]
@@ -187,11 +190,13 @@ class MHDAdjoint:
             self.dev.periodic()
             self.dev.substep(s, self.dt)
             self.dev.swap()
-            t = self.empty()
-            self.store(t)
-            states.append(t)
+            if s == 0:
+                t = self.empty()
+                self.store(t)
+                states.append(t)
         for s in (2, 1, 0):
-            self.load(states[s])
+            if s < 2:
+                self.load(states[s])
             self.dev.periodic()
             ac.check(self.lib.acDeviceIntegrateSubstepAdjoint(
                 self.dev.handle, 0, s, self.dt, ctypes.c_void_p(lam_f.data_ptr()),
