@@ -98,7 +98,16 @@ __device__ __forceinline__ void delay_line_body(const SolveArgs<T>& a, const Adj
     int logical = bid;
     if ((nblocks & 7) == 0)
         logical = (bid & 7) * (nblocks >> 3) + (bid >> 3);
-    const int tile_x = logical % ntx;
+    int tile_x = logical % ntx;
+    // With two workgroups per CU the dispatcher places workgroups b and b + nblocks / 2 on one
+    // CU (observed with per-workgroup clocks at 256^3). The tiles at the x edges, which read
+    // periodic images across the boundary, took 2.5% (plain) to 5.7% (16-bit compensated)
+    // longer, so CUs holding two of them set the launch time. In the second half tile_x ^ 1
+    // pairs each edge tile with an inner one of the same rows: a bijection when each XCD's
+    // half-range is whole rows of an even number of tiles (checked here; otherwise unchanged).
+    if (sizeof(T) == 4 && !ADJ && (ntx & 1) == 0 && (nblocks & 15) == 0 &&
+        (nblocks >> 4) % ntx == 0 && 2 * bid >= nblocks)
+        tile_x ^= 1;
     const int rest = logical / ntx;
     const int tile_y = rest % nty;
     const int chunk = rest / nty;
