@@ -113,14 +113,16 @@ __device__ __forceinline__ gchar* sbase(const void* p)
 {
     return (gchar*)opaque_s64((long long)p);
 }
-// z-chunk of the adjoint's local stage (64 x 4 column blocks): long chunks amortise the six
-// plane steps of each chunk's start-up; the longest of 128 and 64 that still gives at least 512
-// workgroups, else the former choice, the largest of 32, 16 and 8 below nz.
+// z-chunk of the adjoint's local stage (64 x 4 column blocks, one 256-thread workgroup per CU):
+// long chunks amortise the six plane steps of each chunk's start-up; the longest of 256, 128
+// and 64 (up to all of nz) that still gives at least 256 workgroups, one per CU of the MI355X
+// (256^3: one 256-plane chunk per column, 9.20 -> 9.01 ms per adjoint step against 128-plane
+// chunks), else the largest of 32, 16 and 8 below nz.
 inline int adjoint_zchunk(int nx, int ny, int nz)
 {
     const long long tiles = (long long)(nx / 64) * (ny / 4);
-    for (int cand : {128, 64})
-        if (nz % cand == 0 && nz > cand && tiles * (nz / cand) >= 512)
+    for (int cand : {256, 128, 64})
+        if (nz % cand == 0 && nz >= cand && tiles * (nz / cand) >= 256)
             return cand;
     for (int cand : {32, 16, 8})
         if (nz % cand == 0 && nz > cand)
