@@ -139,8 +139,10 @@ __host__ __device__ __forceinline__ double cfl_dt(int mode, double x, double y, 
     return adv < dif ? adv : dif;
 }
 
-// Right-hand side of the eight evolved fields at one cell.
-template <typename T, bool UPWD, bool ENTROPY>
+// Right-hand side of the eight evolved fields at one cell. EXACT (the precise FP32 mode): 1/rho
+// by the correctly rounded division and 1/(rho T) by its own exponential, as in FP64, instead of
+// the reciprocal with one Newton step and 1/(rho T) from the exponentials already taken.
+template <typename T, bool UPWD, bool ENTROPY, bool EXACT = false>
 __device__ __forceinline__ void mhd_rhs(const CellDerivs<T>& d, const PhysParams<T>& P, T rhs[NF])
 {
     const T divu = d.gu[0][0] + d.gu[1][1] + d.gu[2][2];
@@ -168,7 +170,7 @@ __device__ __forceinline__ void mhd_rhs(const CellDerivs<T>& d, const PhysParams
     const T j2 = P.inv_mu0 * (d.gda[2] - d.lapa[2]);
 
     const T rho = ac_exp(d.lr);
-    const T inv_rho = ac_rcp(rho);
+    const T inv_rho = EXACT ? T(1) / rho : ac_rcp(rho);
 
     T cs2;
     T lnT_rel = T(0);  // lnT - lnT0
@@ -212,7 +214,7 @@ __device__ __forceinline__ void mhd_rhs(const CellDerivs<T>& d, const PhysParams
         // T = T0 exp(lnT_rel) and cs2 = cs2_sound exp(lnT_rel), so 1 / (rho T) =
         // pT_scale / (rho cs2) with pT_scale = cs2_sound / T0
         T inv_pT;
-        if constexpr (sizeof(T) == 4)
+        if constexpr (sizeof(T) == 4 && !EXACT)
             inv_pT = inv_rho * P.pT_scale * ac_rcp(cs2);
         else
             inv_pT = ac_exp(-(d.lr + lnT_rel + P.lnT0));

@@ -451,7 +451,7 @@ __device__ __forceinline__ void delay_line_body(const SolveArgs<T>& a, const Adj
             }
             else {
             T rhs[NF];
-            mhd_rhs<T, UPWD, ENTROPY>(d, a.P, rhs);
+            mhd_rhs<T, UPWD, ENTROPY, CANCEL_FREE>(d, a.P, rhs);
             const long long poff = sz * (long long)(z + R);
             T fnew[NF];
             CompRec<KB> cout = crec;  // ss keeps its record entry without entropy
