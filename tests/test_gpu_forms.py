@@ -6,6 +6,8 @@ slow to run; each uses a closed form that holds at any N or an invariant of the 
     P3(-eta k2 dt) per step (k2 the 6th-order second-derivative symbol), and a shear wave by
     P3(-nu k2 dt); both through the multi-step device-dt path, whose solve reads the periodic
     images itself;
+  * the plain-method precision form at 128^3: error bounds of plain, 8-bit and precise FP32
+    after 100 steps against the FP64 mode;
   * 384^3 (the scale form's adjoint size), revolve checkpointing over several steps: the FP64
     gradient of 0.5 |u(T) - u_target|^2 against centred differences of the loss, the gradient
     with a different number of snapshot slots (bit for bit), and the FP32 gradient against the
@@ -158,6 +160,37 @@ def test_scale_form_1024_kahan16(require_gpu):
     lib = ac.load("fp32")
     say("case 1024^3 closed form fp32 kb=16")
     beltrami_check(lib, 1024, 16, steps=2)
+
+
+@pytest.mark.timeout(300)
+def test_precision_form_128_bounds(require_gpu):
+    """The plain-method precision form (16 modes |k| <= 4, amplitude 0.05, 100 steps, dt = 0.2 dx /
+    (cs + max|u|) from the FP64 state) at 128^3: max-rel error per field after 100 steps against
+    the FP64 mode started from the same FP32-rounded state. The precise mode (32-bit records,
+    cancellation-free stencils) holds u and A below 1e-6 (measured u 6.4-6.9e-7, A 3.7-4.1e-7,
+    lnrho 1.07e-6, ss 6.8e-6) and cuts the u, A and lnrho errors of plain FP32 (u 3.7-4.4e-6,
+    A 1.2-1.8e-6, lnrho 4.5e-6) at least two-fold; the 8-bit record also beats plain FP32 in u
+    (2.3-2.9e-6). The bounds leave about 45% over the measured values for a different launch
+    shape's rounding."""
+    import bench
+    say("case precision form 128^3, 100 steps, plain / kahan8 / kahan32 against fp64")
+    r = bench.form_precision(ac, n=128, steps=100, mid=50, seed=2024, kbs=(8, 32), nmodes=16,
+                             kmax=4, ampl=0.05, perturb=0.0, with_va=False)
+    groups = {"lnrho": ["VTXBUF_LNRHO"], "u": ["VTXBUF_UUX", "VTXBUF_UUY", "VTXBUF_UUZ"],
+              "A": ["VTXBUF_AX", "VTXBUF_AY", "VTXBUF_AZ"], "ss": ["VTXBUF_ENTROPY"]}
+    err = {run: {g: max(rec["errors_after_100"][q]["maxrel"] for q in qs)
+                 for g, qs in groups.items()}
+           for run, rec in r["runs"].items() if "errors_after_100" in rec}
+    for run, e in err.items():
+        say(f"  {run}: " + " ".join(f"{g} {v:.3g}" for g, v in e.items()))
+    # the input rounding alone (FP64 from the unrounded state) stays far below every FP32 error
+    assert max(err["fp64_unrounded_ic"].values()) < 1e-6
+    precise, plain = err["fp32_kahan32"], err["fp32"]
+    assert precise["u"] < 1e-6 and precise["A"] < 6e-7
+    assert precise["lnrho"] < 1.6e-6 and precise["ss"] < 1e-5
+    for g in ("u", "A", "lnrho"):
+        assert precise[g] < 0.5 * plain[g]
+    assert err["fp32_kahan8"]["u"] < plain["u"]
 
 
 def _adjoint_modules():
