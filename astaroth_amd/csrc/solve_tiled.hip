@@ -78,8 +78,9 @@ __device__ __forceinline__ void delay_line_body(const SolveArgs<T>& a, const Adj
     // released. The barrier's wait then finds the previous step's stores and the next plane's
     // DMA long done, instead of waiting for the stores just issued (vmcnt cannot tell loads from
     // stores on this target, so the wait before the barrier covers both). Used in FP32 by the
-    // adjoint's local stage (55 stores per cell) and by the forward kernels up to 16-bit
-    // compensation (plain FP32 0.598 -> 0.584 ms per launch at 256^3, 16-bit 0.68 -> 0.66 ms);
+    // adjoint's local stage (47 derivative adjoints, lambda_w and lambda_f stored per cell) and
+    // by the forward kernels up to 16-bit compensation (plain FP32 0.598 -> 0.584 ms per launch
+    // at 256^3, 16-bit 0.68 -> 0.66 ms);
     // the 32-bit compensated and FP64 kernels would spill the longer-lived registers.
     constexpr bool LATE = sizeof(T) == 4 && (ADJ || KB < 32);
     // Skip the in-plane stage (3) on the planes that are not output. Measured: FP32 plain, 8- and
