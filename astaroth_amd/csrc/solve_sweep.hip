@@ -347,7 +347,7 @@ __device__ __forceinline__ void sweep_body(const SolveArgs<T>& a, int zchunk, in
                 d.laps = dv[9] + dzz[7];
 
                 T rhs[NF];
-                mhd_rhs<T, UPWD, ENTROPY>(d, a.P, rhs);
+                mhd_rhs<T, UPWD, ENTROPY, CANCEL_FREE>(d, a.P, rhs);
                 const long long poff = sz * (long long)p;  // plane z's array index is p - R + R
                 T fnew[NF];
                 const CompRec<KB>& crec = cq[u & 1];
