@@ -14,21 +14,36 @@ __device__ __forceinline__ T der1(const T* __restrict__ f, long long c, long lon
     return k[1] * (f[c + s] - f[c - s]) + k[2] * (f[c + 2 * s] - f[c - 2 * s]) +
            k[3] * (f[c + 3 * s] - f[c - 3 * s]);
 }
-template <typename T>
+// Second (and upwind) derivative. CF: the cancellation-free form of the precise 32-bit mode,
+// sum_o k_o ((f_o - f_0) + (f_-o - f_0)), whose weights sum to zero like the sum form's, so the
+// centre weight drops out; differences of neighbouring values are exact in FP32.
+template <bool CF, typename T>
 __device__ __forceinline__ T der2(const T* __restrict__ f, long long c, long long s, const T* k)
 {
+    if constexpr (CF) {
+        const T f0 = f[c];
+        return k[1] * ((f[c + s] - f0) + (f[c - s] - f0)) +
+               k[2] * ((f[c + 2 * s] - f0) + (f[c - 2 * s] - f0)) +
+               k[3] * ((f[c + 3 * s] - f0) + (f[c - 3 * s] - f0));
+    }
     return k[0] * f[c] + k[1] * (f[c + s] + f[c - s]) + k[2] * (f[c + 2 * s] + f[c - 2 * s]) +
            k[3] * (f[c + 3 * s] + f[c - 3 * s]);
 }
-template <typename T>
+// Mixed derivative; CF pairs the corners as (f(o,o) - f(o,-o)) - (f(-o,o) - f(-o,-o)).
+template <bool CF, typename T>
 __device__ __forceinline__ T derm(const T* __restrict__ f, long long c, long long sa, long long sb,
                                   const T* k)
 {
     T r = T(0);
 #pragma unroll
-    for (int o = 1; o <= 3; ++o)
-        r += k[o] * ((f[c + o * sa + o * sb] + f[c - o * sa - o * sb]) -
-                     (f[c + o * sa - o * sb] + f[c - o * sa + o * sb]));
+    for (int o = 1; o <= 3; ++o) {
+        if constexpr (CF)
+            r += k[o] * ((f[c + o * sa + o * sb] - f[c + o * sa - o * sb]) -
+                         (f[c - o * sa + o * sb] - f[c - o * sa - o * sb]));
+        else
+            r += k[o] * ((f[c + o * sa + o * sb] + f[c - o * sa - o * sb]) -
+                         (f[c + o * sa - o * sb] + f[c - o * sa + o * sb]));
+    }
     return r;
 }
 
@@ -39,6 +54,8 @@ __global__ void __launch_bounds__(256) solve_direct_kernel(SolveArgs<T> a)
     const int j = threadIdx.y + blockIdx.y * blockDim.y;
     const int k = blockIdx.z;
     const bool active = i < a.nx && j < a.ny && k < a.nz;
+    // the precise mode (32-bit compensation records, FP32)
+    constexpr bool CF = sizeof(T) == 4 && KB == 32;
     double u2 = 0.0;  // max |u_new|^2 (the courant timestep mode)
     const T dt = solve_dt(a);
     if (active) {
@@ -55,14 +72,14 @@ __global__ void __launch_bounds__(256) solve_direct_kernel(SolveArgs<T> a)
 #pragma unroll
             for (int ax = 0; ax < 3; ++ax)
                 grad[q][ax] = der1(f, c, st[ax], C.d1[ax]);
-            lap[q] = der2(f, c, sx, C.d2[0]) + der2(f, c, sy, C.d2[1]) + der2(f, c, sz, C.d2[2]);
+            lap[q] = der2<CF>(f, c, sx, C.d2[0]) + der2<CF>(f, c, sy, C.d2[1]) + der2<CF>(f, c, sz, C.d2[2]);
         }
         if (UPWD) {
 #pragma unroll
             for (int q = 0; q < 4; ++q)
 #pragma unroll
                 for (int ax = 0; ax < 3; ++ax)
-                    upw[q][ax] = der2(a.in[q], c, st[ax], C.up[ax]);
+                    upw[q][ax] = der2<CF>(a.in[q], c, st[ax], C.up[ax]);
         }
         T val[NF];
 #pragma unroll
@@ -104,13 +121,16 @@ __global__ void __launch_bounds__(256) solve_direct_kernel(SolveArgs<T> a)
             const T* fy = a.in[2 + 3 * v];
             const T* fz = a.in[3 + 3 * v];
             T* gd = v == 0 ? d.gdu : d.gda;
-            gd[0] = der2(fx, c, sx, C.d2[0]) + derm(fy, c, sx, sy, C.mx[0]) + derm(fz, c, sx, sz, C.mx[1]);
-            gd[1] = derm(fx, c, sx, sy, C.mx[0]) + der2(fy, c, sy, C.d2[1]) + derm(fz, c, sy, sz, C.mx[2]);
-            gd[2] = derm(fx, c, sx, sz, C.mx[1]) + derm(fy, c, sy, sz, C.mx[2]) + der2(fz, c, sz, C.d2[2]);
+            gd[0] = der2<CF>(fx, c, sx, C.d2[0]) + derm<CF>(fy, c, sx, sy, C.mx[0]) +
+                    derm<CF>(fz, c, sx, sz, C.mx[1]);
+            gd[1] = derm<CF>(fx, c, sx, sy, C.mx[0]) + der2<CF>(fy, c, sy, C.d2[1]) +
+                    derm<CF>(fz, c, sy, sz, C.mx[2]);
+            gd[2] = derm<CF>(fx, c, sx, sz, C.mx[1]) + derm<CF>(fy, c, sy, sz, C.mx[2]) +
+                    der2<CF>(fz, c, sz, C.d2[2]);
         }
 
         T rhs[NF];
-        mhd_rhs<T, UPWD, ENTROPY>(d, a.P, rhs);
+        mhd_rhs<T, UPWD, ENTROPY, CF>(d, a.P, rhs);
 
         T fnew[NF];
 #pragma unroll
