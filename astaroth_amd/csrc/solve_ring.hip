@@ -48,6 +48,9 @@ __global__ void __launch_bounds__(TX* TY, (sizeof(T) == 4 ? 2 : 1))
     constexpr int LPS = NCHUNK * 64 / WPE;       // LDS stride of one field-plane (elements)
     constexpr int NR = (NCHUNK + NW - 1) / NW;   // chunks per wave per field-plane (at most)
     constexpr int SLOT = NF * LPS;               // elements per ring slot
+    // 32-bit compensation (the precise mode): second, upwind and mixed derivatives from
+    // differences to the centre value, exact for neighbouring values, instead of neighbour sums
+    constexpr bool CANCEL_FREE = sizeof(T) == 4 && KB == 32;
     static_assert(NT % 64 == 0, "workgroup must be whole waves");
     static_assert((NCHUNK * 64) % WPE == 0, "field-plane stride");
     __shared__ T lds[NSLOT * SLOT];
@@ -185,9 +188,9 @@ __global__ void __launch_bounds__(TX* TY, (sizeof(T) == 4 ? 2 : 1))
                 const T xp = at0(f, i, 0), xm = at0(f, -i, 0);
                 const T yp = at0(f, 0, i), ym = at0(f, 0, -i);
                 Dx[i] = xp - xm;
-                Sx[i] = xp + xm;
+                Sx[i] = CANCEL_FREE ? (xp - c0) + (xm - c0) : xp + xm;
                 Dy[i] = yp - ym;
-                Sy[i] = yp + ym;
+                Sy[i] = CANCEL_FREE ? (yp - c0) + (ym - c0) : yp + ym;
             }
             if (f >= 1 && f <= 6) {
                 const int v = f < 4 ? 0 : 1;
@@ -222,12 +225,19 @@ __global__ void __launch_bounds__(TX* TY, (sizeof(T) == 4 ? 2 : 1))
                     }
                 }
             }
-            const T dxx = C.d2[0][0] * c0 + C.d2[0][1] * Sx[1] + C.d2[0][2] * Sx[2] + C.d2[0][3] * Sx[3];
-            const T dyy = C.d2[1][0] * c0 + C.d2[1][1] * Sy[1] + C.d2[1][2] * Sy[2] + C.d2[1][3] * Sy[3];
+            // the centre weight drops out of the cancellation-free form (the weights sum to zero)
+            auto sym = [&](const T* k, const T* S2) -> T {
+                if constexpr (CANCEL_FREE)
+                    return k[1] * S2[1] + k[2] * S2[2] + k[3] * S2[3];
+                else
+                    return k[0] * c0 + k[1] * S2[1] + k[2] * S2[2] + k[3] * S2[3];
+            };
+            const T dxx = sym(C.d2[0], Sx);
+            const T dyy = sym(C.d2[1], Sy);
             T upxy = T(0);
             if (UPWD && f < 4) {
-                const T upx = C.up[0][0] * c0 + C.up[0][1] * Sx[1] + C.up[0][2] * Sx[2] + C.up[0][3] * Sx[3];
-                const T upy = C.up[1][0] * c0 + C.up[1][1] * Sy[1] + C.up[1][2] * Sy[2] + C.up[1][3] * Sy[3];
+                const T upx = sym(C.up[0], Sx);
+                const T upy = sym(C.up[1], Sy);
                 upxy = aux * upx + auy * upy;
             }
             const bool need_dx = (f != 4);
@@ -283,8 +293,14 @@ __global__ void __launch_bounds__(TX* TY, (sizeof(T) == 4 ? 2 : 1))
             T mxx = T(0), myy = T(0);
 #pragma unroll
             for (int i = 1; i <= 3; ++i) {
-                mxx += C.mx[0][i] * ((at0(fx, i, i) + at0(fx, -i, -i)) - (at0(fx, i, -i) + at0(fx, -i, i)));
-                myy += C.mx[0][i] * ((at0(fy, i, i) + at0(fy, -i, -i)) - (at0(fy, i, -i) + at0(fy, -i, i)));
+                if constexpr (CANCEL_FREE) {
+                    mxx += C.mx[0][i] * ((at0(fx, i, i) - at0(fx, i, -i)) - (at0(fx, -i, i) - at0(fx, -i, -i)));
+                    myy += C.mx[0][i] * ((at0(fy, i, i) - at0(fy, i, -i)) - (at0(fy, -i, i) - at0(fy, -i, -i)));
+                }
+                else {
+                    mxx += C.mx[0][i] * ((at0(fx, i, i) + at0(fx, -i, -i)) - (at0(fx, i, -i) + at0(fx, -i, i)));
+                    myy += C.mx[0][i] * ((at0(fy, i, i) + at0(fy, -i, -i)) - (at0(fy, i, -i) + at0(fy, -i, i)));
+                }
             }
             gip[3 * v + 0] += myy;
             gip[3 * v + 1] += mxx;
@@ -315,12 +331,20 @@ __global__ void __launch_bounds__(TX* TY, (sizeof(T) == 4 ? 2 : 1))
 #pragma unroll
                 for (int i = 1; i <= 3; ++i) {
                     D[i] = fp[i] - qb[f][3 - i];
-                    S[i] = fp[i] + qb[f][3 - i];
+                    S[i] = CANCEL_FREE ? (fp[i] - cen[f]) + (qb[f][3 - i] - cen[f])
+                                       : fp[i] + qb[f][3 - i];
                 }
                 dz[f] = C.d1[2][1] * D[1] + C.d1[2][2] * D[2] + C.d1[2][3] * D[3];
-                dzz[f] = C.d2[2][0] * cen[f] + C.d2[2][1] * S[1] + C.d2[2][2] * S[2] + C.d2[2][3] * S[3];
-                if (UPWD && f < 4)
-                    upz[f] = C.up[2][0] * cen[f] + C.up[2][1] * S[1] + C.up[2][2] * S[2] + C.up[2][3] * S[3];
+                if constexpr (CANCEL_FREE) {
+                    dzz[f] = C.d2[2][1] * S[1] + C.d2[2][2] * S[2] + C.d2[2][3] * S[3];
+                    if (UPWD && f < 4)
+                        upz[f] = C.up[2][1] * S[1] + C.up[2][2] * S[2] + C.up[2][3] * S[3];
+                }
+                else {
+                    dzz[f] = C.d2[2][0] * cen[f] + C.d2[2][1] * S[1] + C.d2[2][2] * S[2] + C.d2[2][3] * S[3];
+                    if (UPWD && f < 4)
+                        upz[f] = C.up[2][0] * cen[f] + C.up[2][1] * S[1] + C.up[2][2] * S[2] + C.up[2][3] * S[3];
+                }
                 if (f & 1)
                     __builtin_amdgcn_sched_barrier(0);
             }
@@ -383,7 +407,7 @@ __global__ void __launch_bounds__(TX* TY, (sizeof(T) == 4 ? 2 : 1))
             d.laps = s_lap + dzz[7];
 
             T rhs[NF];
-            mhd_rhs<T, UPWD, ENTROPY>(d, a.P, rhs);
+            mhd_rhs<T, UPWD, ENTROPY, CANCEL_FREE>(d, a.P, rhs);
             const long long poff = sz * (long long)(z + R);
             T fnew[NF];
             CompRec<KB> cout = crec;  // ss keeps its record entry without entropy

@@ -112,12 +112,13 @@ def test_tiled_vs_direct_128(lib, shape):
         assert e["maxrel"] < tol, (q, e)
 
 
-def test_precise_direct_matches_tiled(require_gpu):
+def test_precise_mode_every_kernel(require_gpu):
     """The precise mode (32-bit records) takes the cancellation-free derivative forms and the
-    exact reciprocals in both the delay-line and the one-cell-per-thread kernel. On a smooth
-    128^3 state after 4 steps, u of both sits equally close to FP64 (measured max-rel 5.8e-7 for
-    each), about a third of the 16-bit mode's error (1.7e-6), which keeps the sum forms."""
-    say("case precise mode, delay-line vs one-cell-per-thread kernel, 128^3, 4 steps")
+    exact reciprocals in the delay-line, the one-cell-per-thread and the ring kernel. On a smooth
+    128^3 state after 4 steps, u of each sits equally close to FP64 (measured max-rel 5.8e-7 for
+    the first two), about a third of the 16-bit mode's error (1.7e-6), which keeps the sum
+    forms."""
+    say("case precise mode, delay-line / one-cell-per-thread / ring kernel, 128^3, 4 steps")
     n = (128, 128, 128)
     l32, l64 = ac.load("fp32"), ac.load("fp64")
     data = initial(l32, physics_info(l32, n, (64, 4, 64)), seed=5, perturb=False)
@@ -127,15 +128,19 @@ def test_precise_direct_matches_tiled(require_gpu):
                 steps, expect_shape=(64, 4, 64))
     direct = run(l32, physics_info(l32, n, (-1, 0, 0), AC_kahan_bits=32), data, np.float32(dt),
                  steps, expect_shape=(0, 0, 0))
+    ring = run(l32, physics_info(l32, n, (32, 8, 64), AC_kahan_bits=32), data, np.float32(dt),
+               steps, expect_shape=(32, 8, 64))
     k16 = run(l32, physics_info(l32, n, (-1, 0, 0), AC_kahan_bits=16), data, np.float32(dt),
               steps, expect_shape=(0, 0, 0))
     eu = {}
-    for name, x in (("tiled32", tiled), ("direct32", direct), ("direct16", k16)):
+    for name, x in (("tiled32", tiled), ("direct32", direct), ("ring32", ring),
+                    ("direct16", k16)):
         e = O.field_errors(ref, x)
         eu[name] = max(e[q]["maxrel"] for q in (1, 2, 3))
         say(f"  {name}: " + " ".join(f"{ac.FIELD_NAMES[q]} {e[q]['maxrel']:.3g}" for q in range(8)))
-    assert eu["direct32"] < 1.3 * eu["tiled32"]
-    assert eu["direct32"] < 0.6 * eu["direct16"]
+    for name in ("direct32", "ring32"):
+        assert eu[name] < 1.3 * eu["tiled32"], name
+        assert eu[name] < 0.6 * eu["direct16"], name
 
 
 @pytest.mark.parametrize("kb,shape", [(8, (64, 4, 32)), (16, (64, 4, 32)), (32, (64, 4, 32)),
